@@ -1,0 +1,219 @@
+/*
+ * xflat.h -- C-ABI of the MI355X multi-angle evolution step.
+ *
+ * The reference exposes this path as the compile-time plugin namespace
+ * nbgrp::PHY_MODULE (include/Phy.h:18-161, selected by global.h:5-19) over the
+ * NBeam class (include/NBeam.h:57-189), driven by nmr::evolutionLoop
+ * (src/Nmr.cpp:89-262).  This library replaces it with plain C entry points:
+ * opaque context handles that own device-resident structure-of-arrays beam
+ * storage, int status returns (no exceptions cross the ABI) and plain
+ * pointers/sizes only.
+ *
+ * Host arrays of wave functions always use the reference's NetCDF snapshot
+ * layout psi[beam][prtcl][comp][ebin] (src/IO_f.cpp:276-292), where
+ * beam = theta*Pbins + phi (MA: Pbins = 1), prtcl in {nu_e, anti-nu_e, nu_x,
+ * anti-nu_x} (NBeam.h:19-22 and nbm_prtcls = 2*Flvs, NBeam.cpp:149-152) and
+ * comp in {Re a, Im a, Re b, Im b} (NBeam::psi, NBeam.h:108-124).
+ */
+#ifndef XFLAT_H_
+#define XFLAT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define XF_OK          0
+#define XF_EINVAL     -1   /* bad argument or unsupported configuration        */
+#define XF_EHIP       -2   /* a HIP runtime call failed (message: xf_last_error) */
+#define XF_ENOMEM     -3   /* device or host allocation failed                  */
+#define XF_ESTATE     -4   /* state not SU(2)-partner symmetric for 2-species   */
+#define XF_EIO        -5   /* file could not be read or written                 */
+#define XF_EBARRIER   -6   /* a grid barrier did not complete (bounded wait)    */
+#define XF_ENODEV     -7   /* no HIP device available                           */
+
+/* ---- geometry modules (reference: PHY_MODULE choice, global.h:5-19) ---- */
+#define XF_MA   0          /* bulb multi-angle, src/Phy_MA.cpp                  */
+#define XF_MAA  1          /* multi-azimuthal-angle, src/Phy_MAA.cpp            */
+
+/* ---- precision of the fused stepper ------------------------------------ */
+#define XF_FP64   0        /* FP64 state and arithmetic (parity mode)           */
+#define XF_MIXED  1        /* FP64 state; FP32 propagator increments;
+                              FP64 moments, error norm, accept/reject          */
+#define XF_FP32   2        /* FP32 state and propagator; FP64 moments, error    */
+
+/*
+ * Physics and run parameters, in the reference's internal units (after the
+ * unit scaling of src/Parser.cpp:74-123).  xf_parse_config fills it from a
+ * config*.txt file; unset keys are zero, as for the reference's
+ * zero-initialised Util globals (src/Util.cpp:10-78).
+ */
+typedef struct xf_params {
+    int    module;                 /* XF_MA or XF_MAA                          */
+    int    abins, pbins, ebins;    /* Abins= Pbins= Ebins=                     */
+    int    flvs;                   /* Flvs= (only 2 is supported, as upstream) */
+    int    spoints;                /* SPoints= (parsed, unused by MA/MAA)      */
+    double E0, E1;                 /* E0= E1= [1/km]  (MeV * MEV)              */
+    double R0, Rn, dr, max_dr;     /* R0= Rn= dr= max_dr= [km]                 */
+    double eps0, ksi, mu;          /* eps0= ksi= mu=                           */
+    int    Ts, Tn;                 /* Ts= step budget, Tn= wall-time budget [s] */
+    double theta, theta2;          /* theta= ; theta2 = 2*theta                 */
+    double dm2;                    /* dm2= [1/km^2] (eV^2 * EV2)                */
+    double L[4];                   /* Lve= Lv_e= Lvt= Lv_t= (erg/s * ERG_S)     */
+    double T[4];                   /* Tve= Tv_e= Tvt= Tv_t= (MeV * MEV)         */
+    double eta[4];                 /* eta_ve= eta_v_e= eta_vt= eta_v_t=         */
+    int    has_matter;             /* hasMatter=                               */
+    double Ye, Rv, Mns, gs, S, hNS;/* Ye= Rv= Mns= gs= S= hNS=                  */
+    double nb0;                    /* nb0= [1/km^3] (cm^-3 * 1e15)              */
+    /* IO and orchestration keywords (src/Parser.cpp:40-60) */
+    int    dump_mode;              /* dumpMode= bit0 snapshot, bit1 probability */
+    int    new_file_step, sync_step;
+    double r_step1; int t_step1, itr_step1;
+    double r_step2; int t_step2, itr_step2;
+    int    start_beam, end_beam;
+    int    multi_node_bench, min_nodes, max_nodes;
+    char   file_prefix[256];
+} xf_params;
+
+/* Launch/precision options of the fused stepper. Zero = automatic. */
+typedef struct xf_opts {
+    int precision;        /* XF_FP64 / XF_MIXED / XF_FP32                       */
+    int species;          /* 0 auto, 2 = SU(2)-partner reduced, 4 = explicit    */
+    int renorm;           /* unitarity renormalisation of accepted states
+                             (-1 auto: on for XF_MIXED/XF_FP32, off for FP64)   */
+    int block;            /* threads per workgroup (64..1024), 0 auto          */
+    int cpt;              /* energy cells per thread per tile (1,2,4), 0 auto   */
+    int resident;         /* 1 keep state in registers across steps, 0 stream,
+                             -1 auto                                            */
+    int grid;             /* workgroups (streaming mode), 0 auto                */
+} xf_opts;
+
+/* One attempted radial step of the fused stepper (Nmr.cpp:153-249). */
+typedef struct xf_step_rec {
+    double  r;          /* radius at the start of the attempt                  */
+    double  dr;         /* step length attempted                               */
+    double  err;        /* sqrt(max over beams of sum_e fv |psi4-psi1|^2)      */
+    int32_t accepted;   /* err < eps0                                          */
+    int32_t pad;
+} xf_step_rec;
+
+/* Result of one xf_run_steps / xf_ensemble_run_steps call. */
+typedef struct xf_run_info {
+    int64_t steps;          /* attempted steps in this call                    */
+    int64_t accepted;       /* accepted steps in this call                     */
+    int64_t total_steps;    /* counter_tot since the state was set             */
+    int64_t total_accepted; /* counter since the state was set                 */
+    double  r, dr;          /* radius and next step length after the call      */
+    double  last_err;       /* error of the last attempted step                */
+    double  kernel_ms;      /* device time of the call (HIP events)            */
+    int32_t stop_reason;    /* 0 budget, 1 r reached Rn, 2 dump due, 3 error   */
+    int32_t pad;
+} xf_run_info;
+
+/* Stop conditions for xf_run_steps beyond the step budget. */
+typedef struct xf_stop {
+    double dump_rstep;     /* >0: stop after an accepted step with
+                              r - dump_last_r > dump_rstep or r == Rn
+                              (IO trigger, src/IO_f.cpp:402)                   */
+    double dump_last_r;
+    int    dump_itrstep;   /* >0: stop after an accepted step whose attempt
+                              index it satisfies it % dump_itrstep == 0        */
+    int    pad;
+} xf_stop;
+
+typedef struct xf_ctx xf_ctx;
+typedef struct xf_ensemble xf_ensemble;
+
+/* ---- library ------------------------------------------------------------ */
+const char* xf_last_error(void);                 /* thread-local message     */
+int  xf_version(void);
+int  xf_device_count(int* n);
+
+/* Config file reader, keywords and unit scaling of Parser::readLines
+ * (src/Parser.cpp:25-127).  Unknown keys are ignored, '#' starts a comment. */
+int  xf_parse_config(const char* path, xf_params* out);
+/* Same, from an in-memory text. */
+int  xf_parse_config_text(const char* text, xf_params* out);
+
+/* ---- context: replaces phy::init / nmr::init / nbm::init ----------------
+ * xf_create runs the set-up of nbm::init (NBeam.cpp:138-209: spectra fv,
+ * H_vac), phy::init (Phy_MA.cpp:233-284, Phy_MAA.cpp:247-304: angle grids,
+ * hvv coefficient, L/<E>), matt::init (Matt.cpp:22-29) and allocates the four
+ * beam sets of nmr::init (Nmr.cpp:48-70) on the device, initialised to pure
+ * flavour states like NBeam::NBeam(int) (NBeam.cpp:236-277).               */
+int  xf_create(const xf_params* p, const xf_opts* o, xf_ctx** out);
+int  xf_destroy(xf_ctx* ctx);                    /* phy::freemem, Phy_MA.cpp:290-302 */
+
+/* Dimension queries, Phy.h:24-67. */
+int  xf_beam_len(const xf_ctx* ctx);             /* phy::beamLen               */
+int  xf_get_dim(const xf_ctx* ctx);              /* phy::getDim (5 MA, 6 MAA)   */
+int  xf_count_dim(const xf_ctx* ctx, size_t* count);   /* phy::countDim        */
+
+/* Host-side set-up values (for checks and output headers). */
+int  xf_get_spectra(const xf_ctx* ctx, double* fv /*4*NE*/, double* hvac /*2*NE*/,
+                    double* L_E /*4*/, double* Ev /*4*/);
+int  xf_get_geometry(const xf_ctx* ctx, double* cos2R /*Abins*/, double* coef /*1*/);
+/* matt::getHm (Matt.cpp:58-64). */
+int  xf_get_hm(const xf_ctx* ctx, double r, double* hm);
+
+/* ---- fine-grained Phy.h mirror on the four beam sets ---------------------
+ * Beam set ids 0..3 stand for nubm_0, nubm_1, nubm_3, nubm_4 (Nmr.cpp:42-46);
+ * hvv ids 0..3 for h0, h1, h3, h4 (Nmr.cpp:105-109).  All FP64, 4 species.  */
+int  xf_beams_upload(xf_ctx* ctx, int set, const double* psi);
+int  xf_beams_download(xf_ctx* ctx, int set, double* psi);
+int  xf_beams_init_pure(xf_ctx* ctx, int set);                  /* NBeam(int)        */
+int  xf_calc_esum(xf_ctx* ctx, int set);                        /* NBeam::calcESum, NBeam.cpp:812-840 */
+int  xf_get_esum(xf_ctx* ctx, int set, double* out /*beamLen*4*/);  /* NBeam::getESum */
+int  xf_calc_angle_bins(xf_ctx* ctx, double r, int pt);         /* Phy.h:108        */
+int  xf_calc_delta_ls(xf_ctx* ctx, double dr, int cur, int s, int e); /* Phy.h:117 */
+int  xf_calc_hvv(xf_ctx* ctx, int set, int pt, int hvv);        /* Phy.h:125        */
+int  xf_get_hvv(xf_ctx* ctx, int hvv, double* out /*beams*4*/);
+int  xf_evolve(xf_ctx* ctx, int in, int pt, int hvv, double hm, int out);      /* Phy.h:147 */
+int  xf_evolve_hvv(xf_ctx* ctx, int in, int pt, int hvv, double hm, int out);  /* Phy.h:148 */
+int  xf_evolve_avg(xf_ctx* ctx, int in, int pt, int hvv, double hm, int out, int avg); /* Phy.h:149 */
+int  xf_evolve_avg_inplace(xf_ctx* ctx, int pt, int hvv, double hm, int io, int avg); /* Phy.h:152 */
+int  xf_avg_beam(xf_ctx* ctx, int in, int out);                 /* Phy.h:159        */
+/* max over beams of NBeam::calcErrHvv(a vs b) (NBeam.cpp:903-938, loop
+ * Nmr.cpp:195-206), returned before the square root; re-caches ESum(a).     */
+int  xf_calc_err_hvv(xf_ctx* ctx, int a, int b, double* max_err);
+
+/* ---- fused stepper: replaces nmr::evolutionLoop (Nmr.cpp:89-262) --------
+ * The state lives on the device in the precision of xf_opts.precision.
+ * xf_state_upload takes a full 4-species psi (reference layout) plus the
+ * radius and step length; with 2-species mode the nu_x / anti-nu_x columns
+ * must be SU(2) partners (-conj(b), conj(a)) of nu_e / anti-nu_e, else
+ * XF_ESTATE.  xf_state_download reconstructs all 4 species.               */
+int  xf_state_upload(xf_ctx* ctx, const double* psi, double r, double dr);
+int  xf_state_init_pure(xf_ctx* ctx);            /* pure flavour, r = R0, dr = dr= */
+int  xf_state_download(xf_ctx* ctx, double* psi, double* r, double* dr);
+int  xf_state_counters(const xf_ctx* ctx, int64_t* total, int64_t* accepted);
+int  xf_state_species(const xf_ctx* ctx);        /* 2 or 4 in use           */
+/* Attempt up to max_steps radial steps (fewer if r reaches Rn or a stop
+ * condition fires).  log (optional) receives one record per attempt.       */
+int  xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop,
+                  xf_step_rec* log, xf_run_info* info);
+/* Launch shape in use (after automatic choice). */
+int  xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_tile);
+/* Project autotuner: times candidate (block, cpt, resident) shapes for
+ * `steps` steps each on a scratch copy of the state and keeps the fastest. */
+int  xf_autotune(xf_ctx* ctx, int steps, xf_opts* best, double* best_ms_per_step);
+
+/* ---- batched ensemble: many independent instances, one persistent launch -
+ * Every instance has its own grid, spectra, matter profile, r, dr and
+ * accept/reject; workgroups are partitioned between instances.            */
+int  xf_ensemble_create(const xf_params* p, int n, const xf_opts* o, xf_ensemble** out);
+int  xf_ensemble_destroy(xf_ensemble* ens);
+int  xf_ensemble_upload(xf_ensemble* ens, int i, const double* psi, double r, double dr);
+int  xf_ensemble_init_pure(xf_ensemble* ens, int i);
+int  xf_ensemble_download(xf_ensemble* ens, int i, double* psi, double* r, double* dr);
+int  xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info /*n*/);
+int  xf_ensemble_cells(const xf_ensemble* ens, int64_t* cells);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XFLAT_H_ */

@@ -1,0 +1,131 @@
+"""Generates the golden fixtures tests/golden/*.npz from the reference itself.
+
+Runs the reference's evolution path, compiled from its unmodified sources by
+oracle/Makefile into oracle/_ref/ (with the project's raw-file IO plugin
+oracle/ref_io_recorder.cpp), on small configurations:
+
+* MA cases start from a seeded perturbed state injected through the
+  reference's resume path (fio::fillInitData, src/IO_f.cpp:197-250, which sets
+  R0, dr and psi);
+* MAA cases start from pure flavour states (the reference's MAA resume and
+  perturbation are switched off, Phy_MAA.cpp:281,412).
+
+Every accepted step is recorded by the IO hook of Nmr.cpp:238 as
+(it = attempt index, r after the step, dr used); the wave functions are kept
+for a few records.  Run in the build container (the reference is not on the
+GPU host):  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from xflat_amd.config import config_text  # noqa: E402
+from xflat_amd.seed import seeded_psi, pure_psi  # noqa: E402
+
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+# name -> (config overrides, seed or None for pure states, amplitude, records to keep)
+CASES = {
+    # configs[0] physics (bulb MA, inverted hierarchy) at R_nu, small grid:
+    # the first attempts with dr = 1e-3 km are rejected until dr ~ 4e-8 km.
+    "ma_r10_ih": (dict(Abins=32, Ebins=16, R0=10, Ts=400, dm2="-2.4E-3"), 11, 1e-6, 6),
+    "ma_r10_nh": (dict(Abins=32, Ebins=16, R0=10, Ts=300, dm2="2.4E-3"), 12, 1e-6, 6),
+    # further out: larger steps, tighter tolerance (configs[1] uses eps0 = 1e-9)
+    "ma_r50_ih": (dict(Abins=48, Ebins=24, R0=50, Ts=300, dm2="-2.4E-3", eps0="1.E-9"), 13, 1e-6, 6),
+    # no matter term (hasMatter= 0), vacuum + nu-nu only
+    "ma_vac": (dict(Abins=16, Ebins=16, R0=30, Ts=200, hasMatter=0), 14, 1e-6, 6),
+    # MAA: 4-moment kernel, pure flavour initial states
+    "maa_r50": (dict(Abins=8, Pbins=6, Ebins=12, R0=50, Ts=200), None, 0.0, 6),
+    "maa_r10": (dict(Abins=8, Pbins=4, Ebins=12, R0=10, Ts=200, dm2="2.4E-3"), None, 0.0, 6),
+}
+
+
+def _run_case(name, over, seed, amp, keep, isa="v3"):
+    over = dict(over)
+    maa = int(over.get("Pbins", 1)) > 1
+    exe = os.path.join(REF_DIR, f"xflat_ref_{'maa' if maa else 'ma'}_{isa}")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} missing: run `make -C oracle ref` first")
+    txt = config_text(**over)
+    nth = int(over["Abins"]); nph = int(over.get("Pbins", 1)); ne = int(over["Ebins"])
+    nbeam = nth * nph
+    r0 = float(over["R0"]); dr0 = 0.001
+    if seed is not None:
+        psi0 = seeded_psi(nbeam, ne, seed, amp)
+    else:
+        psi0 = pure_psi(nbeam, ne)
+    with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else None) as td:
+        cfg = os.path.join(td, "config.txt")
+        with open(cfg, "w") as f:
+            f.write(txt)
+        out = os.path.join(td, "rec.bin")
+        env = dict(os.environ, XFLAT_REF_OUT=out, XFLAT_REF_PSI_EVERY="1", OMP_NUM_THREADS="4")
+        args = [exe, cfg]
+        if seed is not None:
+            init = os.path.join(td, "init.bin")
+            with open(init, "wb") as f:
+                np.array([nbeam * 4, 4, ne], dtype=np.int64).tofile(f)
+                np.array([r0, dr0], dtype=np.float64).tofile(f)
+                psi0.astype(np.float64).tofile(f)
+            env["XFLAT_REF_INIT"] = init
+            args.append(init)
+        res = subprocess.run(args, env=env, capture_output=True, text=True, cwd=td, timeout=600)
+        if res.returncode != 0:
+            raise SystemExit(f"{name}: reference failed\n{res.stdout}\n{res.stderr}")
+        m = re.search(r"Total iterations:\s*(\d+)\s+Correct iterations:\s*(\d+)\s+at radius\(Km\):\s*([0-9.eE+-]+)", res.stdout)
+        tot, acc = int(m.group(1)), int(m.group(2))
+        raw = np.fromfile(out, dtype=np.uint8)
+    hdr = raw[:32].view(np.int64)
+    nnb, ncmp, nebin = int(hdr[0]), int(hdr[1]), int(hdr[2])
+    per = nnb * ncmp * nebin
+    pos = 32
+    its, rs, drs, psis = [], [], [], []
+    while pos < raw.size:
+        it = raw[pos:pos + 8].view(np.int64)[0]
+        r, dr = raw[pos + 8:pos + 24].view(np.float64)
+        has = raw[pos + 24:pos + 32].view(np.int64)[0]
+        pos += 32
+        its.append(it); rs.append(r); drs.append(dr)
+        if has:
+            psis.append(raw[pos:pos + 8 * per].view(np.float64).reshape(nbeam, 4, 4, ne).copy())
+            pos += 8 * per
+        else:
+            psis.append(None)
+    nrec = len(its)
+    sel = sorted(set(np.linspace(0, nrec - 1, keep).round().astype(int).tolist())) if nrec else []
+    return dict(
+        config=np.array(txt), module=np.int64(1 if maa else 0),
+        psi0=psi0, r0=np.float64(r0), dr0=np.float64(dr0),
+        it=np.array(its, dtype=np.int64), r=np.array(rs), dr=np.array(drs),
+        psi_idx=np.array(sel, dtype=np.int64),
+        psi=np.stack([psis[i] for i in sel]) if sel else np.zeros((0, nbeam, 4, 4, ne)),
+        total=np.int64(tot), accepted=np.int64(acc),
+        meta=np.array(json.dumps(dict(case=name, seed=seed, amp=amp, isa=isa,
+                                      source="oracle/_ref reference build"))),
+    )
+
+
+def main(names=None):
+    for name, (over, seed, amp, keep) in CASES.items():
+        if names and name not in names:
+            continue
+        d = _run_case(name, over, seed, amp, keep)
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **d)
+        print(f"{name}: {int(d['total'])} attempts, {int(d['accepted'])} accepted, "
+              f"r -> {d['r'][-1] if d['r'].size else d['r0']:.9f}, {os.path.getsize(path)//1024} KiB")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
