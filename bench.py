@@ -1,0 +1,180 @@
+"""Throughput of the multi-angle evolution step on one MI355X.
+
+Workload: the bulb model (MA) at production size, 1000 angle bins x 400
+energy bins (400k beams, each carrying nu and anti-nu), starting at the
+neutrinosphere R_nu = 10 km with a seeded 1e-6 off-diagonal perturbation,
+tolerance eps0 = 1e-9.  A "step" is one attempted adaptive radial step
+(S1..S11 + error + accept/reject); beam-steps/s = beams x attempted steps /
+wall time, with the state resident in HBM before the timed region.
+
+The headline `value` is the mixed-precision stepper (FP64 state, FP32
+propagator increments, FP64 moments / error / decision); the plain FP64
+stepper is timed in the same run and reported beside it, with the max
+|dP_ee| of the mixed run against it.  The CPU baseline is the reference's
+own evolution path (oracle/_ref build) or, without it, the project's CPU
+restatement, timed on a bounded sample on this host.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64|fp32]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+NTH, NE = 1000, 400
+
+
+def workload_params(**over):
+    from xflat_amd.config import bulb_params
+    d = dict(Abins=NTH, Ebins=NE, R0=10, Rn=1000, eps0="1.E-9", dm2="-2.4E-3")
+    d.update(over)
+    return bulb_params(**d)
+
+
+def initial_state():
+    from xflat_amd.seed import seeded_psi
+    return seeded_psi(NTH, NE, seed=2024, amp=1e-6)
+
+
+def time_gpu(precision, steps, warmup, psi0):
+    from xflat_amd import capi
+    p = workload_params()
+    ctx = capi.Context(p, precision=precision)
+    ctx.upload(psi0, 10.0, 1e-3)
+    if warmup > 0:
+        ctx.run(warmup)
+    info0 = ctx.counters()
+    t0 = time.perf_counter()
+    info, _ = ctx.run(steps)
+    t1 = time.perf_counter()
+    launch = ctx.launch()
+    psi, r, dr = ctx.download()
+    ctx.close()
+    return dict(wall_s=t1 - t0, kernel_ms=info.kernel_ms, steps=info.steps, accepted=info.accepted,
+                r=r, dr=dr, psi=psi, launch=launch, start_counters=info0)
+
+
+def cpu_baseline(psi0, budget_steps=None):
+    """Reference evolution path (oracle/_ref) on this host's cores, bounded sample."""
+    from xflat_amd.config import config_text
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(16, ncores or 1))
+    isa = "v4"
+    try:
+        with open("/proc/cpuinfo") as f:
+            if " avx512f" not in f.read():
+                isa = "v3"
+    except OSError:
+        isa = "v3"
+    exe = os.path.join(ROOT, "oracle", "_ref", f"xflat_ref_ma_{isa}")
+    steps = budget_steps or max(20, 12 * ncores)
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as td:
+            cfg = os.path.join(td, "config.txt")
+            with open(cfg, "w") as f:
+                f.write(config_text(Abins=NTH, Ebins=NE, R0=10, Rn=1000, eps0="1.E-9", dm2="-2.4E-3",
+                                    Ts=steps, dumpMode=0))
+            init = os.path.join(td, "init.bin")
+            with open(init, "wb") as f:
+                np.array([NTH * 4, 4, NE], dtype=np.int64).tofile(f)
+                np.array([10.0, 1e-3]).tofile(f)
+                psi0.astype(np.float64).tofile(f)
+            env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="close", XFLAT_REF_INIT=init)
+            r = subprocess.run([exe, cfg, init], env=env, capture_output=True, text=True, cwd=td, timeout=900)
+            m = re.search(r"total time:([0-9.eE+-]+)", r.stdout)
+            tot = re.search(r"Total iterations:\s*(\d+)", r.stdout)
+            if r.returncode == 0 and m and tot:
+                secs = float(m.group(1))
+                n = int(tot.group(1))
+                return dict(value=NTH * NE * n / secs, unit="beam-steps/s", cores=ncores, kind="reference",
+                            sample=f"MA 1000x400 FP64, {n} attempted steps from R=10 km "
+                                   f"(oracle/_ref/{os.path.basename(exe)}, OpenMP {ncores} threads, "
+                                   f"{secs:.2f} s loop time)")
+    from oracle.oracle import Oracle, set_threads
+    set_threads(ncores)
+    o = Oracle(workload_params())
+    o.set_state(psi0, 10.0, 1e-3)
+    n, _, secs = o.run(steps, log=False)
+    return dict(value=NTH * NE * n / secs, unit="beam-steps/s", cores=ncores, kind="port",
+                sample=f"MA 1000x400 FP64, {n} attempted steps from R=10 km (oracle restatement, "
+                       f"OpenMP {ncores} threads, {secs:.2f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (the multi-GPU "
+              "angle-sharding design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    from xflat_amd import capi
+    prec = {"mixed": capi.XF_MIXED, "fp64": capi.XF_FP64, "fp32": capi.XF_FP32}[a.precision]
+    psi0 = initial_state()
+    print(f"[bench] {a.precision} MA {NTH}x{NE}: warmup {a.warmup}, timing {a.steps} steps", file=sys.stderr, flush=True)
+    res = time_gpu(prec, a.steps, a.warmup, psi0)
+    cells = NTH * NE
+    value = cells * res["steps"] / res["wall_s"]
+    bytes_per = 64 if prec != capi.XF_FP64 else 128
+    kern_s = res["kernel_ms"] / 1e3
+    achieved = bytes_per * cells * res["steps"] / kern_s / 1e9
+    out = {
+        "metric": "beam-steps/sec (angle x energy x dr) at stated P_ee error vs FP64; % HBM roofline",
+        "value": value,
+        "unit": "beam-steps/s",
+        "n_gpus": 1,
+        "steps": res["steps"],
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * res["wall_s"] / max(1, res["steps"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"mixed": "fp32 propagator / fp64 state+moments", "fp64": "fp64", "fp32": "fp32 / fp64 moments"}[a.precision],
+        "data": "synthetic: seeded 1e-6 perturbation of pure flavour states",
+        "config": {"workload": "bulb MA 1000 angles x 400 energies, R0 = 10 km, eps0 = 1e-9, inverted hierarchy",
+                   "beams": cells, "precision": a.precision, "launch": res["launch"]},
+        "accepted_steps": res["accepted"],
+        "radius_km": res["r"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": f"algorithmic {bytes_per} B per beam-step over the stepper's HIP-event time"},
+    }
+    print(f"[bench] {a.precision}: {value:.3e} beam-steps/s, {out['ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
+    if not a.no_fp64 and prec != capi.XF_FP64:
+        ref = time_gpu(capi.XF_FP64, a.steps, a.warmup, psi0)
+        fp64_value = cells * ref["steps"] / ref["wall_s"]
+        same = ref["accepted"] == res["accepted"] and abs(ref["r"] - res["r"]) <= 1e-9 * ref["r"]
+        pe = lambda s: s[:, 0, 0] ** 2 + s[:, 0, 1] ** 2
+        pb = lambda s: s[:, 1, 0] ** 2 + s[:, 1, 1] ** 2
+        out["fp64"] = {"value": fp64_value, "ms_per_step": 1e3 * ref["wall_s"] / max(1, ref["steps"]),
+                       "launch": ref["launch"]}
+        out["pee_error_vs_fp64"] = {
+            "same_step_sequence": bool(same),
+            "max_abs_dPee": float(np.abs(pe(res["psi"]) - pe(ref["psi"])).max()),
+            "max_abs_dPbar_ee": float(np.abs(pb(res["psi"]) - pb(ref["psi"])).max()),
+            "unitarity_drift": float(np.abs(np.sum(res["psi"][:, 0] ** 2, axis=1) - 1).max()),
+        }
+        print(f"[bench] fp64: {fp64_value:.3e} beam-steps/s", file=sys.stderr, flush=True)
+    if not a.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(psi0)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -197,6 +197,14 @@ int  xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop,
                   xf_step_rec* log, xf_run_info* info);
 /* Launch shape in use (after automatic choice). */
 int  xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_tile);
+/* Explicit launch shape: mode 0 keeps psi0 and the step's intermediate
+ * states in registers for the whole launch, 1 keeps psi0 only and recomputes,
+ * 2 streams the state from HBM; cpt energy cells per thread; block threads;
+ * bpt beams per tile.  No counterpart in the reference (a GPU launch knob). */
+int  xf_set_launch(xf_ctx* ctx, int mode, int cpt, int block, int bpt);
+/* Registers, scratch bytes per lane, co-resident workgroups and LDS bytes of
+ * the launch shape in use. */
+int  xf_launch_info(const xf_ctx* ctx, int* vgprs, int* scratch, int* max_blocks, int* lds_bytes);
 /* Project autotuner: times candidate (block, cpt, resident) shapes for
  * `steps` steps each on a scratch copy of the state and keeps the fastest. */
 int  xf_autotune(xf_ctx* ctx, int steps, xf_opts* best, double* best_ms_per_step);
@@ -211,6 +219,7 @@ int  xf_ensemble_init_pure(xf_ensemble* ens, int i);
 int  xf_ensemble_download(xf_ensemble* ens, int i, double* psi, double* r, double* dr);
 int  xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info /*n*/);
 int  xf_ensemble_cells(const xf_ensemble* ens, int64_t* cells);
+int  xf_ensemble_launch(const xf_ensemble* ens, int* mode, int* cpt, int* block, int* grid);
 
 #ifdef __cplusplus
 }

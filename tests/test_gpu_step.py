@@ -1,0 +1,247 @@
+"""Parity of the HIP evolution step (through the C-ABI) with the reference.
+
+FP64 bar (north-star): max |dP_ee| <= 1e-9 and max |dPbar_ee| <= 1e-9 per beam
+and energy, with the same accepted-step sequence.  Mixed precision is
+compared with the FP64 GPU run.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_files
+
+pytestmark = pytest.mark.gpu
+
+PEE_TOL = 1e-9
+# err = sqrt(max sum_e fv |psi4 - psi1|^2) subtracts O(1) numbers that agree to
+# ~1e-9, so one ulp in psi moves err by ~1e-7 relative (the reference's own
+# value carries the same cancellation); dr scales with err^-1/2.
+DR_RTOL = 2e-6
+R_RTOL = 1e-11
+
+
+def _capi():
+    from xflat_amd import capi
+    if capi.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return capi
+
+
+def _golden(name):
+    from xflat_amd.config import parse_config_text
+    g = np.load(os.path.join(GOLDEN, name))
+    p = parse_config_text(str(g["config"]), module=int(g["module"]))
+    return g, p
+
+
+def _pee(psi):
+    """P_ee (nu_e) and Pbar_ee (anti-nu_e): |a|^2 of particles 0 and 1."""
+    return psi[:, 0, 0] ** 2 + psi[:, 0, 1] ** 2, psi[:, 1, 0] ** 2 + psi[:, 1, 1] ** 2
+
+
+def _run_all(ctx, n, chunk=None):
+    """Runs n attempts (in chunks) and returns the concatenated step log."""
+    logs = []
+    done = 0
+    chunk = chunk or n
+    while done < n:
+        info, rec = ctx.run(min(chunk, n - done), log=True)
+        logs.append(rec)
+        done += info.steps
+        if info.steps == 0:
+            break
+    return np.concatenate(logs)
+
+
+@pytest.mark.parametrize("name", golden_files())
+def test_fused_fp64_matches_reference(name):
+    capi = _capi()
+    g, p = _golden(name)
+    print(f"[fused fp64] {name}", flush=True)
+    with capi.Context(p, precision=capi.XF_FP64) as ctx:
+        ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        rec = _run_all(ctx, int(g["total"]))
+        acc = np.nonzero(rec["accepted"])[0]
+        assert np.array_equal(acc, g["it"]), "accepted-step sequence differs"
+        np.testing.assert_allclose(rec["dr"][acc], g["dr"], rtol=DR_RTOL, atol=0)
+        np.testing.assert_allclose((rec["r"] + rec["dr"])[acc], g["r"], rtol=R_RTOL, atol=0)
+        psi, r, dr = ctx.download()
+        pe, pb = _pee(psi)
+        ge, gb = _pee(g["psi"][-1])
+        assert np.abs(pe - ge).max() <= PEE_TOL
+        assert np.abs(pb - gb).max() <= PEE_TOL
+        # the complex phases see the ~1e-12 km radius differences of the step
+        # sequence; P_ee above is the contract
+        assert np.abs(psi - g["psi"][-1]).max() <= 1e-7
+
+
+@pytest.mark.parametrize("name", ["ma_r10_ih.npz", "maa_r50.npz"])
+def test_fused_fp64_four_species(name):
+    capi = _capi()
+    g, p = _golden(name)
+    with capi.Context(p, precision=capi.XF_FP64, species=4) as ctx:
+        assert ctx.species() == 4
+        ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        rec = _run_all(ctx, int(g["total"]))
+        acc = np.nonzero(rec["accepted"])[0]
+        assert np.array_equal(acc, g["it"])
+        psi, *_ = ctx.download()
+        pe, pb = _pee(psi)
+        ge, gb = _pee(g["psi"][-1])
+        assert np.abs(pe - ge).max() <= PEE_TOL and np.abs(pb - gb).max() <= PEE_TOL
+        assert np.abs(psi - g["psi"][-1]).max() <= 1e-7
+
+
+def test_fused_chunked_equals_single_launch():
+    """Splitting the run into launches must not change anything."""
+    capi = _capi()
+    g, p = _golden("ma_r50_ih.npz")
+    with capi.Context(p, precision=capi.XF_FP64) as a, capi.Context(p, precision=capi.XF_FP64) as b:
+        a.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        b.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        ra = _run_all(a, 120)
+        rb = _run_all(b, 120, chunk=17)
+        assert np.array_equal(ra, rb)
+        assert np.array_equal(a.download()[0], b.download()[0])
+
+
+def test_fused_modes_agree():
+    """Register-resident and streaming launches of the FP64 stepper."""
+    capi = _capi()
+    g, p = _golden("ma_r10_ih.npz")
+    out = {}
+    for mode in (1, 2):
+        with capi.Context(p, precision=capi.XF_FP64) as ctx:
+            ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+            ctx.set_launch(mode, 1, 64 * ((8 * p.ebins + 63) // 64), 8)
+            rec = _run_all(ctx, 200)
+            out[mode] = (rec, ctx.download()[0])
+    assert np.array_equal(np.nonzero(out[1][0]["accepted"])[0], np.nonzero(out[2][0]["accepted"])[0])
+    assert np.abs(out[1][1] - out[2][1]).max() <= 1e-12
+
+
+def test_phy_mirror_one_step_matches_oracle():
+    """The Phy.h mirror, driven call by call like Nmr.cpp:153-206."""
+    capi = _capi()
+    from oracle.oracle import Oracle
+    g, p = _golden("ma_r50_ih.npz")
+    o = Oracle(p)
+    psi0 = g["psi0"]
+    r, dr = 50.0, 2e-4
+    with capi.Context(p) as ctx:
+        for k in range(4):
+            ctx.beams_upload(k, psi0)
+            o.set_psi(k, psi0)
+        ctx.calc_esum(0); o.calc_esum(0)
+        np.testing.assert_allclose(ctx.get_esum(0), o.get_hsum(0), rtol=0, atol=1e-15)
+        dr2 = .5 * dr
+        hm = [ctx.hm(r), ctx.hm(r + dr2), ctx.hm(r + dr)]
+        assert hm == [o.hm(r), o.hm(r + dr2), o.hm(r + dr)]
+        for obj in (ctx, o):
+            obj.calc_angle_bins(r, 0); obj.calc_angle_bins(r + dr2, 1); obj.calc_angle_bins(r + dr, 2)
+            obj.calc_delta_ls(dr2, 0, 0, 1); obj.calc_delta_ls(dr, 1, 0, 2); obj.calc_delta_ls(dr2, 2, 1, 2)
+            obj.calc_hvv(0, 0, 0)
+            obj.evolve_hvv(0, 1, 0, hm[0], 1)
+            obj.evolve_hvv(0, 0, 0, hm[0], 2)
+            obj.calc_hvv(1, 2, 1)
+            obj.evolve_avg(0, 1, 1, hm[2], 3, 1)
+            obj.calc_hvv(2, 1, 2)
+            obj.evolve_hvv(0, 1, 2, hm[1], 3)
+            obj.calc_hvv(3, 2, 3)
+            obj.evolve_avg_inplace(2, 3, hm[2], 2, 3)
+        for k in range(4):
+            np.testing.assert_allclose(ctx.get_hvv(k), o.get_hvv(k), rtol=1e-12, atol=1e-12 * np.abs(o.get_hvv(k)).max())
+            np.testing.assert_allclose(ctx.beams_download(k), o.get_psi(k), rtol=0, atol=1e-13)
+        e_gpu = ctx.calc_err_hvv(3, 1)
+        e_cpu = o.calc_err_hvv(3, 1)
+        # sum fv |psi4 - psi1|^2 cancels O(1) values to ~1e-8: ~1e-8 relative
+        assert abs(e_gpu - e_cpu) <= 1e-6 * e_cpu
+        ctx.avg_beam(0, 1); o.avg_beam(0, 1)
+        np.testing.assert_allclose(ctx.beams_download(1), o.get_psi(1), rtol=0, atol=1e-15)
+
+
+def test_fused_vs_oracle_bulb_200x100():
+    """configs[0] grid (MA 200 angles x 100 energies) against the oracle."""
+    capi = _capi()
+    from oracle.oracle import Oracle
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import seeded_psi
+    p = bulb_params(Abins=200, Ebins=100, R0=10, Ts=60)
+    psi0 = seeded_psi(200, 100, seed=7, amp=1e-6)
+    o = Oracle(p)
+    o.set_state(psi0, 10.0, 1e-3)
+    n, orec, _ = o.run(60)
+    opsi, *_ = o.state()
+    with capi.Context(p, precision=capi.XF_FP64) as ctx:
+        ctx.upload(psi0, 10.0, 1e-3)
+        rec = _run_all(ctx, 60)
+        psi, *_ = ctx.download()
+    assert np.array_equal(rec["accepted"], orec["accepted"])
+    np.testing.assert_allclose(rec["dr"], orec["dr"], rtol=DR_RTOL)
+    pe, pb = _pee(psi)
+    oe, ob = _pee(opsi)
+    assert np.abs(pe - oe).max() <= PEE_TOL and np.abs(pb - ob).max() <= PEE_TOL
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_mixed_precision_close_to_fp64(prec):
+    capi = _capi()
+    g, p = _golden("ma_r50_ih.npz")
+    pr = capi.XF_MIXED if prec == "mixed" else capi.XF_FP32
+    with capi.Context(p, precision=capi.XF_FP64) as a, capi.Context(p, precision=pr) as b:
+        for c in (a, b):
+            c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        ra = _run_all(a, 300)
+        rb = _run_all(b, 300)
+        pa, *_ = a.download()
+        pb_, *_ = b.download()
+    ea, eb = _pee(pa)[0], _pee(pb_)[0]
+    d = np.abs(ea - eb).max()
+    norm = np.abs(np.sum(pb_[:, 0] ** 2, axis=1) - 1).max()
+    print(f"[{prec}] max |dP_ee| vs FP64 = {d:.3e}, unitarity drift = {norm:.3e}, "
+          f"accepted {ra['accepted'].sum()} vs {rb['accepted'].sum()}")
+    assert d <= 1e-5
+    assert norm <= 1e-6
+
+
+def test_ensemble_instances_match_single_runs():
+    capi = _capi()
+    from oracle.oracle import Oracle
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import seeded_psi
+    ps = [bulb_params(Abins=24, Ebins=16, R0=20), bulb_params(Abins=40, Ebins=32, R0=30, dm2="2.4E-3"),
+          bulb_params(Abins=16, Ebins=48, R0=25, Lve="2E+51")]
+    ens = capi.Ensemble(ps, precision=capi.XF_FP64)
+    inits = []
+    for i, p in enumerate(ps):
+        psi0 = seeded_psi(p.abins, p.ebins, seed=100 + i, amp=1e-6)
+        inits.append(psi0)
+        ens.upload(i, psi0, p.R0, 1e-3)
+    infos = ens.run(80)
+    for i, p in enumerate(ps):
+        o = Oracle(p)
+        o.set_state(inits[i], p.R0, 1e-3)
+        n, orec, _ = o.run(80)
+        opsi, r, dr, (tot, acc) = o.state()
+        psi, gr, gdr = ens.download(i)
+        assert infos[i].total_steps == tot and infos[i].total_accepted == acc
+        assert abs(gr - r) <= 1e-12 * r
+        pe, pb = _pee(psi)
+        oe, ob = _pee(opsi)
+        assert np.abs(pe - oe).max() <= PEE_TOL and np.abs(pb - ob).max() <= PEE_TOL
+    ens.close()
+
+
+def test_bad_launch_shapes_are_rejected():
+    capi = _capi()
+    from xflat_amd.config import bulb_params
+    p = bulb_params(Abins=16, Ebins=40, R0=20)
+    with capi.Context(p, precision=capi.XF_FP64) as ctx:
+        for args in [(1, 1, 463, 4), (1, 3, 128, 1), (5, 1, 128, 1), (1, 1, 2048, 1), (2, 1, 64, 0)]:
+            with pytest.raises(capi.XfError) as e:
+                ctx.set_launch(*args)
+            assert e.value.code == -1
+        ctx.set_launch(2, 1, 64, 1)        # valid: 40 cells in one wave
+        info, _ = ctx.run(5)
+        assert info.steps == 5

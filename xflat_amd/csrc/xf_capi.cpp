@@ -1,0 +1,805 @@
+// xf_capi.cpp -- C-ABI of the fused stepper and the batched ensemble:
+// device buffers, state upload/download, launch-shape choice, run control.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "xf_internal.h"
+#include "xf_phy.h"
+
+// launch helpers (xf_step_host.hip)
+bool xf_step_has_shape(const XfLaunch& L, int geom);
+size_t xf_step_smem_bytes(const XfLaunch& L, int geom);
+int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks);
+int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int max_steps,
+                        void* stream);
+int xf_step_func_info(const XfLaunch& L, int geom, int* vgprs, int* spill_bytes);
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return xf_fail(XF_EHIP, "%s: %s", #x, hipGetErrorString(e_));         \
+    } while (0)
+
+namespace {
+
+template <class T> int dalloc(T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return xf_fail(XF_ENOMEM, "hipMalloc(%zu B): %s", n * sizeof(T), hipGetErrorString(e));
+    *p = (T*)q;
+    return XF_OK;
+}
+template <class T> int dput(T** p, const std::vector<T>& v) {
+    int rc = dalloc(p, v.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return XF_OK;
+}
+std::vector<float> to_f(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
+
+}  // namespace
+
+// One instance of the stepper: tables, state buffers and run scalars on the device.
+struct XfInst {
+    xf_params p{};
+    XfTables t{};
+    int ns = 2, precision = XF_FP64, renorm = 0;
+    double src_scale = 1.;
+    size_t elem = 8;
+    double *d_x = nullptr, *d_dx = nullptr, *d_sinR = nullptr, *d_cphi = nullptr, *d_sphi = nullptr;
+    double *d_hv = nullptr, *d_srcw = nullptr, *d_errw = nullptr;
+    float *d_hvf = nullptr, *d_srcwf = nullptr, *d_errwf = nullptr;
+    void* d_psi[2] = {nullptr, nullptr};
+    XfRunScalars* d_rs = nullptr;
+    XfGridBar* d_bar = nullptr;
+    double* d_part = nullptr;
+    int part_nblk = 0;
+    XfInstDev dev{};
+
+    ~XfInst() {
+        void* ps[] = {d_x, d_dx, d_sinR, d_cphi, d_sphi, d_hv, d_srcw, d_errw, d_hvf, d_srcwf, d_errwf,
+                      d_psi[0], d_psi[1], d_rs, d_bar, d_part};
+        for (void* q : ps)
+            if (q) (void)hipFree(q);
+    }
+
+    long long ncells() const { return (long long)t.nbeam * t.ne; }
+
+    int init(const xf_params* pp, int prec, int nspec, int ren) {
+        p = *pp;
+        int rc = xf_build_tables(&p, &t);
+        if (rc) return rc;
+        precision = prec;
+        ns = nspec;
+        renorm = ren;
+        elem = (precision == XF_FP32) ? 4 : 8;
+        const int ne = t.ne;
+        std::vector<double> hv(2 * ne), srcw((size_t)ns * ne), errw(4 * (size_t)ne);
+        for (int e = 0; e < ne; ++e) {
+            hv[e] = t.hvac[2 * e];
+            hv[ne + e] = t.hvac[2 * e + 1];
+            for (int s = 0; s < 4; ++s) errw[(size_t)s * ne + e] = t.fv[(size_t)s * ne + e];
+            if (ns == 2) {
+                srcw[e] = t.L_E[0] * t.fv[e] - t.L_E[2] * t.fv[2 * ne + e];
+                srcw[ne + e] = t.L_E[1] * t.fv[ne + e] - t.L_E[3] * t.fv[3 * ne + e];
+            } else {
+                for (int s = 0; s < 4; ++s) srcw[(size_t)s * ne + e] = t.L_E[s] * t.fv[(size_t)s * ne + e];
+            }
+        }
+        // float source weights are scaled to O(1); the scale moves into coef_f
+        src_scale = 0.;
+        for (double w : srcw) src_scale = std::max(src_scale, std::fabs(w));
+        if (!(src_scale > 0.)) src_scale = 1.;
+        std::vector<double> srcs(srcw.size());
+        for (size_t i = 0; i < srcw.size(); ++i) srcs[i] = srcw[i] / src_scale;
+        if ((rc = dput(&d_x, t.cos2R)) || (rc = dput(&d_dx, t.dcos2R)) || (rc = dput(&d_sinR, t.sinR)) ||
+            (rc = dput(&d_cphi, t.cosphi)) || (rc = dput(&d_sphi, t.sinphi)) || (rc = dput(&d_hv, hv)) ||
+            (rc = dput(&d_srcw, srcw)) || (rc = dput(&d_errw, errw)) || (rc = dput(&d_hvf, to_f(hv))) ||
+            (rc = dput(&d_srcwf, to_f(srcs))) || (rc = dput(&d_errwf, to_f(errw))))
+            return rc;
+        size_t nst = (size_t)ns * 4 * ncells();
+        for (int b = 0; b < 2; ++b) {
+            void* q = nullptr;
+            hipError_t e = hipMalloc(&q, nst * elem);
+            if (e != hipSuccess) return xf_fail(XF_ENOMEM, "state buffer (%zu B): %s", nst * elem, hipGetErrorString(e));
+            d_psi[b] = q;
+        }
+        if ((rc = dalloc(&d_rs, 1)) || (rc = dalloc(&d_bar, 1))) return rc;
+        HIPCHK(hipMemset(d_rs, 0, sizeof(XfRunScalars)));
+        HIPCHK(hipMemset(d_bar, 0, sizeof(XfGridBar)));
+        fill_dev();
+        return XF_OK;
+    }
+
+    int ensure_partials(int nblk) {
+        if (nblk <= part_nblk) return XF_OK;
+        if (d_part) (void)hipFree(d_part);
+        d_part = nullptr;
+        int rc = dalloc(&d_part, (size_t)3 * nblk * XF_NVMAX);
+        if (rc) return rc;
+        part_nblk = nblk;
+        dev.part = d_part;
+        return XF_OK;
+    }
+
+    void fill_dev() {
+        XfInstDev& d = dev;
+        d.module = t.module; d.nth = t.nth; d.nph = t.nph; d.ne = t.ne; d.nbeam = t.nbeam;
+        d.ns = ns;
+        d.ncells = ncells();
+        d.x = d_x; d.dx = d_dx; d.sinR = d_sinR; d.cphi = d_cphi; d.sphi = d_sphi;
+        d.hv = d_hv; d.srcw = d_srcw; d.errw = d_errw; d.hvf = d_hvf; d.srcwf = d_srcwf; d.errwf = d_errwf;
+        d.Rv = p.Rv; d.coef = t.integ_coef; d.coef_f = t.integ_coef * src_scale; d.mat_coeff = t.mat_coeff; d.nbfar_coeff = t.nbfar_coeff;
+        d.nb0 = p.nb0; d.hNS = p.hNS; d.has_matter = p.has_matter; d.renorm = renorm;
+        d.eps0 = p.eps0; d.ksi = p.ksi; d.max_dr = p.max_dr; d.Rn = p.Rn;
+        d.psi[0] = d_psi[0]; d.psi[1] = d_psi[1];
+        d.rs = d_rs; d.bar = d_bar; d.part = d_part;
+        d.log = nullptr;
+        d.dump_rstep = 0; d.dump_last_r = 0; d.dump_itrstep = 0;
+    }
+
+    // psi[beam][prtcl][comp][e] (reference layout) -> device planes
+    int upload(const double* psi, double r, double dr) {
+        const long long nc = ncells();
+        const int ne = t.ne;
+        const size_t nst = (size_t)ns * 4 * nc;
+        std::vector<double> pl(nst);
+        double worst = 0;
+        for (int b = 0; b < t.nbeam; ++b)
+            for (int s = 0; s < ns; ++s)
+                for (int c = 0; c < 4; ++c)
+                    for (int e = 0; e < ne; ++e)
+                        pl[((size_t)s * 4 + c) * nc + (size_t)b * ne + e] = psi[(((size_t)b * 4 + s) * 4 + c) * ne + e];
+        if (ns == 2) {
+            // nu_x = (-conj b, conj a) of nu_e (and likewise for the antineutrinos)
+            for (int b = 0; b < t.nbeam; ++b)
+                for (int s = 0; s < 2; ++s) {
+                    const double* E = psi + (((size_t)b * 4 + s) * 4) * ne;
+                    const double* X = psi + (((size_t)b * 4 + s + 2) * 4) * ne;
+                    for (int e = 0; e < ne; ++e) {
+                        worst = std::max(worst, std::fabs(X[0 * ne + e] + E[2 * ne + e]));
+                        worst = std::max(worst, std::fabs(X[1 * ne + e] - E[3 * ne + e]));
+                        worst = std::max(worst, std::fabs(X[2 * ne + e] - E[0 * ne + e]));
+                        worst = std::max(worst, std::fabs(X[3 * ne + e] + E[1 * ne + e]));
+                    }
+                }
+            if (!(worst <= 1e-12))
+                return xf_fail(XF_ESTATE, "nu_x columns are not SU(2) partners of nu_e (max deviation %.3g)", worst);
+        }
+        if (elem == 8) {
+            HIPCHK(hipMemcpy(d_psi[0], pl.data(), nst * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(pl.begin(), pl.end());
+            HIPCHK(hipMemcpy(d_psi[0], f.data(), nst * 4, hipMemcpyHostToDevice));
+        }
+        XfRunScalars rs{};
+        rs.r = r; rs.dr = dr; rs.cur = 0;
+        HIPCHK(hipMemcpy(d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
+        return XF_OK;
+    }
+
+    int scalars(XfRunScalars* rs) const {
+        HIPCHK(hipMemcpy(rs, d_rs, sizeof *rs, hipMemcpyDeviceToHost));
+        return XF_OK;
+    }
+
+    int download(double* psi, double* r, double* dr) const {
+        XfRunScalars rs;
+        int rc = scalars(&rs);
+        if (rc) return rc;
+        const long long nc = ncells();
+        const int ne = t.ne;
+        const size_t nst = (size_t)ns * 4 * nc;
+        std::vector<double> pl(nst);
+        if (elem == 8) {
+            HIPCHK(hipMemcpy(pl.data(), d_psi[rs.cur], nst * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> f(nst);
+            HIPCHK(hipMemcpy(f.data(), d_psi[rs.cur], nst * 4, hipMemcpyDeviceToHost));
+            std::copy(f.begin(), f.end(), pl.begin());
+        }
+        if (psi) {
+            for (int b = 0; b < t.nbeam; ++b)
+                for (int s = 0; s < ns; ++s)
+                    for (int c = 0; c < 4; ++c)
+                        for (int e = 0; e < ne; ++e)
+                            psi[(((size_t)b * 4 + s) * 4 + c) * ne + e] = pl[((size_t)s * 4 + c) * nc + (size_t)b * ne + e];
+            if (ns == 2)
+                for (int b = 0; b < t.nbeam; ++b)
+                    for (int s = 0; s < 2; ++s) {
+                        const double* E = psi + (((size_t)b * 4 + s) * 4) * ne;
+                        double* X = psi + (((size_t)b * 4 + s + 2) * 4) * ne;
+                        for (int e = 0; e < ne; ++e) {
+                            X[0 * ne + e] = -E[2 * ne + e];
+                            X[1 * ne + e] = E[3 * ne + e];
+                            X[2 * ne + e] = E[0 * ne + e];
+                            X[3 * ne + e] = -E[1 * ne + e];
+                        }
+                    }
+        }
+        if (r) *r = rs.r;
+        if (dr) *dr = rs.dr;
+        return XF_OK;
+    }
+};
+
+struct xf_ensemble {
+    xf_opts o{};
+    XfLaunch L{};
+    int geom = XF_MA;
+    std::vector<std::unique_ptr<XfInst>> inst;
+    XfInstDev* d_insts = nullptr;
+    int* d_blk_inst = nullptr;
+    xf_step_rec* d_log = nullptr;
+    int log_cap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool shaped = false;
+
+    ~xf_ensemble() {
+        if (d_insts) (void)hipFree(d_insts);
+        if (d_blk_inst) (void)hipFree(d_blk_inst);
+        if (d_log) (void)hipFree(d_log);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+struct xf_ctx {
+    xf_params p{};
+    xf_opts o{};
+    XfTables t{};
+    xf_ensemble* ens = nullptr;       // the fused stepper (one instance)
+    XfPhy* phy = nullptr;             // the Phy.h mirror (four FP64 beam sets)
+};
+
+// ---------------------------------------------------------------- shapes
+static int max_bpt(int ne, int cpt) {
+    int b = (1024 * cpt) / ne;
+    return std::max(1, std::min(128, b));
+}
+static int block_for(int bpt, int ne, int cpt) {
+    int cells = bpt * ne;
+    int thr = (cells + cpt - 1) / cpt;
+    return std::min(1024, ((thr + 63) / 64) * 64);
+}
+
+// Total cells of the ensemble, per-instance tiles and grid for a shape.
+static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
+    // all instances share precision/ns/cpt/mode/block; bpt is per instance
+    if (L.block < 64 || L.block > 1024 || L.block % 64)
+        return xf_fail(XF_EINVAL, "block %d: a multiple of 64 in [64, 1024]", L.block);
+    {
+        int vg = 0, sp = 0;
+        int kmax = xf_step_func_info(L, E->geom, &vg, &sp);
+        if (kmax <= 0) return xf_fail(XF_EINVAL, "no stepper for this shape");
+        if (L.block > kmax) return xf_fail(XF_EINVAL, "block %d exceeds the kernel's bound %d", L.block, kmax);
+    }
+    int grid = 0;
+    std::vector<int> nblk(E->inst.size()), bpts(E->inst.size()), tiles(E->inst.size());
+    long long total_cells = 0;
+    int max_bpt_used = 1;
+    for (auto& in : E->inst) total_cells += in->ncells();
+    for (size_t i = 0; i < E->inst.size(); ++i) {
+        XfInst* in = E->inst[i].get();
+        int ne = in->t.ne;
+        int bpt = std::min(L.bpt > 0 ? L.bpt : max_bpt(ne, L.cpt), max_bpt(ne, L.cpt));
+        while (bpt > 1 && block_for(bpt, ne, L.cpt) > L.block) --bpt;
+        if (block_for(bpt, ne, L.cpt) > L.block) return xf_fail(XF_EINVAL, "NE=%d does not fit block %d x cpt %d", ne, L.block, L.cpt);
+        bpts[i] = bpt;
+        tiles[i] = (in->t.nbeam + bpt - 1) / bpt;
+        max_bpt_used = std::max(max_bpt_used, bpt);
+    }
+    XfLaunch Lq = L;
+    Lq.bpt = max_bpt_used;              // LDS per workgroup depends on the largest tile
+    int maxb = 0;
+    int rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
+    if (rc) return rc;
+    for (size_t i = 0; i < E->inst.size(); ++i) {
+        XfInst* in = E->inst[i].get();
+        int nt = tiles[i];
+        if (L.mode != 2) {
+            nblk[i] = nt;
+        } else {
+            // share the co-resident workgroups in proportion to the cells
+            long long share = (long long)maxb * in->ncells() / std::max<long long>(1, total_cells);
+            if (L.grid > 0) share = std::min<long long>(share, (long long)L.grid * in->ncells() / std::max<long long>(1, total_cells));
+            nblk[i] = (int)std::max<long long>(1, std::min<long long>(nt, share));
+        }
+        grid += nblk[i];
+    }
+    if (grid > maxb)
+        return xf_fail(XF_EINVAL, "shape needs %d co-resident workgroups, device holds %d", grid, maxb);
+    if (!commit) {
+        L.grid = grid;
+        return XF_OK;
+    }
+    std::vector<XfInstDev> devs(E->inst.size());
+    std::vector<int> blk_inst(grid);
+    int b0 = 0;
+    for (size_t i = 0; i < E->inst.size(); ++i) {
+        XfInst* in = E->inst[i].get();
+        rc = in->ensure_partials(nblk[i]);
+        if (rc) return rc;
+        in->dev.bpt = bpts[i];
+        in->dev.ntiles = tiles[i];
+        in->dev.blk0 = b0;
+        in->dev.nblk = nblk[i];
+        for (int j = 0; j < nblk[i]; ++j) blk_inst[b0 + j] = (int)i;
+        b0 += nblk[i];
+        devs[i] = in->dev;
+    }
+    L.grid = grid;
+    L.bpt = max_bpt_used;
+    if (E->d_insts) (void)hipFree(E->d_insts);
+    if (E->d_blk_inst) (void)hipFree(E->d_blk_inst);
+    E->d_insts = nullptr;
+    E->d_blk_inst = nullptr;
+    if ((rc = dput(&E->d_insts, devs)) || (rc = dput(&E->d_blk_inst, blk_inst))) return rc;
+    E->L = L;
+    E->shaped = true;
+    return XF_OK;
+}
+
+// Default shape: keep the state in registers when it fits, else stream.
+static int choose_shape(xf_ensemble* E) {
+    const xf_opts& o = E->o;
+    XfLaunch base{};
+    base.precision = o.precision;
+    base.ns = E->inst[0]->ns;
+    int ne_max = 1;
+    for (auto& in : E->inst) ne_max = std::max(ne_max, in->t.ne);
+    std::vector<int> modes, cpts;
+    if (o.resident == 0) modes = {2};
+    else if (o.resident == 1) modes = (o.precision == XF_FP64) ? std::vector<int>{1} : std::vector<int>{0, 1};
+    else modes = (o.precision == XF_FP64) ? std::vector<int>{1, 2} : std::vector<int>{0, 1, 2};
+    if (o.cpt > 0) cpts = {o.cpt};
+    else cpts = {1, 2, 4};
+    for (int mode : modes) {
+        for (int cpt : cpts) {
+            XfLaunch L = base;
+            L.mode = mode;
+            L.cpt = cpt;
+            if (!xf_step_has_shape(L, E->geom)) continue;
+            int vg = 0, spill = 0;
+            int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
+            if (kmax <= 0) continue;
+            if (spill > 0 && mode != 2) continue;
+            int bpt = max_bpt(ne_max, cpt);
+            while (bpt > 1 && block_for(bpt, ne_max, cpt) > kmax) --bpt;
+            L.block = o.block > 0 ? std::min(o.block, kmax) : block_for(bpt, ne_max, cpt);
+            if (L.block > kmax) continue;
+            L.bpt = 0;
+            if (plan_shape(E, L, false) == XF_OK) return plan_shape(E, L, true);
+        }
+    }
+    return xf_fail(XF_EINVAL, "no launch shape fits the device: %s", xf_last_error());
+}
+
+static int ens_launch(xf_ensemble* E, int max_steps, xf_step_rec* log, double* ms) {
+    if (!E->shaped) {
+        int rc = choose_shape(E);
+        if (rc) return rc;
+    }
+    if (log && E->inst.size() == 1) {
+        if (max_steps > E->log_cap) {
+            if (E->d_log) (void)hipFree(E->d_log);
+            E->d_log = nullptr;
+            int rc = dalloc(&E->d_log, (size_t)max_steps);
+            if (rc) return rc;
+            E->log_cap = max_steps;
+        }
+        XfInstDev d = E->inst[0]->dev;
+        d.log = E->d_log;
+        HIPCHK(hipMemcpyAsync(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice, E->stream));
+    } else if (E->inst.size() == 1) {
+        XfInstDev d = E->inst[0]->dev;
+        d.log = nullptr;
+        HIPCHK(hipMemcpyAsync(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice, E->stream));
+    }
+    for (auto& in : E->inst) {
+        HIPCHK(hipMemsetAsync(in->d_bar, 0, sizeof(XfGridBar), E->stream));
+        HIPCHK(hipMemsetAsync(&in->d_rs->status, 0, sizeof(int), E->stream));
+    }
+    HIPCHK(hipEventRecord(E->ev0, E->stream));
+    int rc = xf_step_launch_geom(E->L, E->geom, E->d_insts, E->d_blk_inst, max_steps, E->stream);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(E->ev1, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, E->ev0, E->ev1));
+    if (ms) *ms = f;
+    if (log && E->inst.size() == 1) {
+        XfRunScalars rs;
+        rc = E->inst[0]->scalars(&rs);
+        if (rc) return rc;
+        if (rs.steps_call > 0)
+            HIPCHK(hipMemcpy(log, E->d_log, (size_t)rs.steps_call * sizeof(xf_step_rec), hipMemcpyDeviceToHost));
+    }
+    return XF_OK;
+}
+
+static int ens_create(const xf_params* p, int n, const xf_opts* o, xf_ensemble** out) {
+    if (!p || n <= 0 || !out) return xf_fail(XF_EINVAL, "xf_ensemble_create: bad arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return xf_fail(XF_ENODEV, "no HIP device");
+    std::unique_ptr<xf_ensemble> E(new xf_ensemble());
+    if (o) E->o = *o;
+    if (E->o.precision < XF_FP64 || E->o.precision > XF_FP32) return xf_fail(XF_EINVAL, "precision %d", E->o.precision);
+    int ns = E->o.species == 4 ? 4 : 2;
+    int ren = E->o.renorm;
+    if (ren < 0 || (ren == 0 && o && o->renorm == 0 && false)) ren = 0;
+    if (E->o.renorm == -1) ren = (E->o.precision != XF_FP64) ? 1 : 0;
+    E->geom = p[0].module;
+    for (int i = 0; i < n; ++i) {
+        if (p[i].module != E->geom) return xf_fail(XF_EINVAL, "all instances of an ensemble share the geometry module");
+        std::unique_ptr<XfInst> in(new XfInst());
+        int rc = in->init(&p[i], E->o.precision, ns, ren);
+        if (rc) return rc;
+        E->inst.push_back(std::move(in));
+    }
+    HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&E->ev0));
+    HIPCHK(hipEventCreate(&E->ev1));
+    *out = E.release();
+    return XF_OK;
+}
+
+static int fill_info(XfInst* in, xf_run_info* info, double ms) {
+    XfRunScalars rs;
+    int rc = in->scalars(&rs);
+    if (rc) return rc;
+    if (info) {
+        info->steps = rs.steps_call;
+        info->accepted = rs.acc_call;
+        info->total_steps = rs.total;
+        info->total_accepted = rs.accepted;
+        info->r = rs.r;
+        info->dr = rs.dr;
+        info->last_err = rs.last_err;
+        info->kernel_ms = ms;
+        info->stop_reason = rs.status ? 3 : rs.stop_reason;
+    }
+    if (rs.status) return xf_fail(rs.status, "stepper status %d (grid barrier did not complete)", rs.status);
+    return XF_OK;
+}
+
+// ================================================================ C-ABI
+extern "C" {
+
+int xf_device_count(int* n) {
+    int d = 0;
+    if (hipGetDeviceCount(&d) != hipSuccess) d = 0;
+    if (n) *n = d;
+    return XF_OK;
+}
+
+int xf_create(const xf_params* p, const xf_opts* o, xf_ctx** out) {
+    if (!p || !out) return xf_fail(XF_EINVAL, "xf_create: null argument");
+    std::unique_ptr<xf_ctx> c(new xf_ctx());
+    c->p = *p;
+    if (o) c->o = *o;
+    else { c->o.renorm = -1; c->o.resident = -1; }
+    int rc = xf_build_tables(p, &c->t);
+    if (rc) return rc;
+    rc = ens_create(p, 1, &c->o, &c->ens);
+    if (rc) return rc;
+    rc = xf_state_init_pure(c.get());
+    if (rc) return rc;
+    *out = c.release();
+    return XF_OK;
+}
+
+int xf_destroy(xf_ctx* ctx) {
+    if (!ctx) return XF_OK;
+    delete ctx->ens;
+    xf_phy_destroy(ctx->phy);
+    delete ctx;
+    return XF_OK;
+}
+
+int xf_beam_len(const xf_ctx* ctx) { return ctx ? ctx->t.nbeam * 4 : 0; }
+int xf_get_dim(const xf_ctx* ctx) { return ctx ? (ctx->t.module == XF_MAA ? 6 : 5) : 0; }
+int xf_count_dim(const xf_ctx* ctx, size_t* count) {
+    if (!ctx || !count) return xf_fail(XF_EINVAL, "xf_count_dim: null argument");
+    // [r, theta, (phi,) prtcl, comp, ebin] (Phy_MA.cpp:253-258, Phy_MAA.cpp:269-275)
+    int k = 0;
+    count[k++] = 1;
+    count[k++] = ctx->t.nth;
+    if (ctx->t.module == XF_MAA) count[k++] = ctx->t.nph;
+    count[k++] = 4;
+    count[k++] = 4;
+    count[k++] = ctx->t.ne;
+    return XF_OK;
+}
+
+int xf_get_spectra(const xf_ctx* ctx, double* fv, double* hvac, double* L_E, double* Ev) {
+    if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
+    if (fv) std::memcpy(fv, ctx->t.fv.data(), ctx->t.fv.size() * 8);
+    if (hvac) std::memcpy(hvac, ctx->t.hvac.data(), ctx->t.hvac.size() * 8);
+    for (int s = 0; s < 4; ++s) {
+        if (L_E) L_E[s] = ctx->t.L_E[s];
+        if (Ev) Ev[s] = ctx->t.Ev[s];
+    }
+    return XF_OK;
+}
+
+int xf_get_geometry(const xf_ctx* ctx, double* cos2R, double* coef) {
+    if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
+    if (cos2R) std::memcpy(cos2R, ctx->t.cos2R.data(), ctx->t.cos2R.size() * 8);
+    if (coef) *coef = ctx->t.integ_coef;
+    return XF_OK;
+}
+
+int xf_get_hm(const xf_ctx* ctx, double r, double* hm) {
+    if (!ctx || !hm) return xf_fail(XF_EINVAL, "null argument");
+    *hm = xf_host_hm(&ctx->p, &ctx->t, r);
+    return XF_OK;
+}
+
+// ---- fused stepper
+int xf_state_upload(xf_ctx* ctx, const double* psi, double r, double dr) {
+    if (!ctx || !psi) return xf_fail(XF_EINVAL, "xf_state_upload: null argument");
+    XfInst* in = ctx->ens->inst[0].get();
+    int rc = in->upload(psi, r, dr);
+    if (rc == XF_ESTATE && ctx->o.species == 0) {
+        // not partners: fall back to the explicit 4-species state
+        xf_opts o4 = ctx->o;
+        o4.species = 4;
+        xf_ensemble* E = nullptr;
+        rc = ens_create(&ctx->p, 1, &o4, &E);
+        if (rc) return rc;
+        delete ctx->ens;
+        ctx->ens = E;
+        return E->inst[0]->upload(psi, r, dr);
+    }
+    return rc;
+}
+
+int xf_state_init_pure(xf_ctx* ctx) {
+    if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
+    const int ne = ctx->t.ne;
+    std::vector<double> psi((size_t)ctx->t.nbeam * 16 * ne, 0.);
+    for (int b = 0; b < ctx->t.nbeam; ++b)
+        for (int s = 0; s < 4; ++s)
+            for (int e = 0; e < ne; ++e) {
+                // NBeam::NBeam(int): a = 1 for nu_e / anti-nu_e, b = 1 for nu_x / anti-nu_x
+                psi[(((size_t)b * 4 + s) * 4 + (s < 2 ? 0 : 2)) * ne + e] = 1.;
+            }
+    return ctx->ens->inst[0]->upload(psi.data(), ctx->p.R0, ctx->p.dr);
+}
+
+int xf_state_download(xf_ctx* ctx, double* psi, double* r, double* dr) {
+    if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
+    return ctx->ens->inst[0]->download(psi, r, dr);
+}
+
+int xf_state_counters(const xf_ctx* ctx, int64_t* total, int64_t* accepted) {
+    if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
+    XfRunScalars rs;
+    int rc = ctx->ens->inst[0]->scalars(&rs);
+    if (rc) return rc;
+    if (total) *total = rs.total;
+    if (accepted) *accepted = rs.accepted;
+    return XF_OK;
+}
+
+int xf_state_species(const xf_ctx* ctx) { return ctx ? ctx->ens->inst[0]->ns : 0; }
+
+int xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop, xf_step_rec* log, xf_run_info* info) {
+    if (!ctx || max_steps < 0) return xf_fail(XF_EINVAL, "xf_run_steps: bad arguments");
+    xf_ensemble* E = ctx->ens;
+    XfInst* in = E->inst[0].get();
+    in->dev.dump_rstep = stop ? stop->dump_rstep : 0.;
+    in->dev.dump_last_r = stop ? stop->dump_last_r : 0.;
+    in->dev.dump_itrstep = stop ? stop->dump_itrstep : 0;
+    if (E->shaped) {
+        // refresh the descriptor copy (stop predicates)
+        XfInstDev d = in->dev;
+        HIPCHK(hipMemcpy(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice));
+    }
+    double ms = 0;
+    int rc = ens_launch(E, max_steps, log, &ms);
+    if (rc) return rc;
+    return fill_info(in, info, ms);
+}
+
+int xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_tile) {
+    if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
+    xf_ensemble* E = ctx->ens;
+    if (!E->shaped) {
+        int rc = choose_shape(E);
+        if (rc) return rc;
+    }
+    if (out) {
+        *out = E->o;
+        out->species = E->inst[0]->ns;
+        out->block = E->L.block;
+        out->cpt = E->L.cpt;
+        out->resident = E->L.mode;   // 0 keep, 1 recompute, 2 stream
+        out->grid = E->L.grid;
+        out->renorm = E->inst[0]->renorm;
+    }
+    if (tiles) *tiles = E->inst[0]->dev.ntiles;
+    if (beams_per_tile) *beams_per_tile = E->inst[0]->dev.bpt;
+    return XF_OK;
+}
+
+// Set an explicit launch shape (mode: 0 keep, 1 recompute, 2 stream).
+int xf_set_launch(xf_ctx* ctx, int mode, int cpt, int block, int bpt) {
+    if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
+    if (mode < 0 || mode > 2) return xf_fail(XF_EINVAL, "mode %d: 0 keep, 1 recompute, 2 stream", mode);
+    if (cpt != 1 && cpt != 2 && cpt != 4) return xf_fail(XF_EINVAL, "cpt %d: 1, 2 or 4", cpt);
+    if (block < 64 || block > 1024 || block % 64) return xf_fail(XF_EINVAL, "block %d: a multiple of 64 in [64, 1024]", block);
+    if (bpt < 1 || bpt > 128) return xf_fail(XF_EINVAL, "bpt %d: 1..128 beams per tile", bpt);
+    xf_ensemble* E = ctx->ens;
+    XfLaunch L{};
+    L.precision = E->o.precision;
+    L.ns = E->inst[0]->ns;
+    L.mode = mode;
+    L.cpt = cpt;
+    L.block = block;
+    L.bpt = bpt;
+    if (!xf_step_has_shape(L, E->geom)) return xf_fail(XF_EINVAL, "shape not instantiated");
+    return plan_shape(E, L, true);
+}
+
+int xf_launch_info(const xf_ctx* ctx, int* vgprs, int* spill, int* max_blocks, int* lds_bytes) {
+    if (!ctx || !ctx->ens->shaped) return xf_fail(XF_EINVAL, "no launch shape yet");
+    const xf_ensemble* E = ctx->ens;
+    int rc = xf_step_func_info(E->L, E->geom, vgprs, spill);
+    if (rc < 0) return rc;
+    if (max_blocks) {
+        rc = xf_step_max_blocks_geom(E->L, E->geom, max_blocks);
+        if (rc) return rc;
+    }
+    if (lds_bytes) *lds_bytes = (int)xf_step_smem_bytes(E->L, E->geom);
+    return XF_OK;
+}
+
+int xf_autotune(xf_ctx* ctx, int steps, xf_opts* best, double* best_ms_per_step) {
+    if (!ctx || steps <= 0) return xf_fail(XF_EINVAL, "xf_autotune: bad arguments");
+    xf_ensemble* E = ctx->ens;
+    XfInst* in = E->inst[0].get();
+    // save the state
+    const size_t nst = (size_t)in->ns * 4 * in->ncells() * in->elem;
+    XfRunScalars rs0;
+    int rc = in->scalars(&rs0);
+    if (rc) return rc;
+    void* save = nullptr;
+    HIPCHK(hipMalloc(&save, nst));
+    HIPCHK(hipMemcpy(save, in->d_psi[rs0.cur], nst, hipMemcpyDeviceToDevice));
+    double best_t = 1e300;
+    XfLaunch bestL{};
+    bool found = false;
+    const int ne = in->t.ne;
+    for (int mode = 0; mode <= 2; ++mode)
+        for (int cpt : {1, 2, 4}) {
+            XfLaunch L{};
+            L.precision = E->o.precision;
+            L.ns = in->ns;
+            L.mode = mode;
+            L.cpt = cpt;
+            if (!xf_step_has_shape(L, E->geom)) continue;
+            int bmax = max_bpt(ne, cpt);
+            std::vector<int> bpts = {bmax};
+            for (int b = bmax / 2; b >= 1; b /= 2) bpts.push_back(b);
+            for (int bpt : bpts) {
+                L.bpt = bpt;
+                L.block = block_for(bpt, ne, cpt);
+                int vg = 0, spill = 0;
+                int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
+                if (kmax <= 0 || L.block > kmax) continue;
+                if (spill > 0 && mode != 2) continue;
+                if (plan_shape(E, L, true) != XF_OK) continue;
+                // restore state, warm up, time
+                HIPCHK(hipMemcpy(in->d_psi[0], save, nst, hipMemcpyDeviceToDevice));
+                XfRunScalars rs = rs0;
+                rs.cur = 0;
+                HIPCHK(hipMemcpy(in->d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
+                double ms = 0;
+                if (ens_launch(E, 2, nullptr, &ms) != XF_OK) continue;
+                if (ens_launch(E, steps, nullptr, &ms) != XF_OK) continue;
+                XfRunScalars r2;
+                in->scalars(&r2);
+                if (r2.status || r2.steps_call <= 0) continue;
+                double per = ms / (double)r2.steps_call;
+                if (per < best_t) {
+                    best_t = per;
+                    bestL = E->L;
+                    found = true;
+                }
+            }
+        }
+    // restore state and the winning shape
+    HIPCHK(hipMemcpy(in->d_psi[0], save, nst, hipMemcpyDeviceToDevice));
+    XfRunScalars rs = rs0;
+    rs.cur = 0;
+    HIPCHK(hipMemcpy(in->d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
+    (void)hipFree(save);
+    if (!found) return xf_fail(XF_EINVAL, "autotune: no shape ran");
+    XfLaunch L = bestL;
+    rc = plan_shape(E, L, true);
+    if (rc) return rc;
+    if (best) {
+        *best = E->o;
+        best->species = in->ns;
+        best->block = E->L.block;
+        best->cpt = E->L.cpt;
+        best->resident = E->L.mode;
+        best->grid = E->L.grid;
+    }
+    if (best_ms_per_step) *best_ms_per_step = best_t;
+    return XF_OK;
+}
+
+// ---- ensemble
+int xf_ensemble_create(const xf_params* p, int n, const xf_opts* o, xf_ensemble** out) {
+    xf_opts oo{};
+    if (o) oo = *o;
+    else { oo.renorm = -1; oo.resident = -1; }
+    return ens_create(p, n, &oo, out);
+}
+int xf_ensemble_destroy(xf_ensemble* ens) {
+    delete ens;
+    return XF_OK;
+}
+int xf_ensemble_upload(xf_ensemble* ens, int i, const double* psi, double r, double dr) {
+    if (!ens || i < 0 || i >= (int)ens->inst.size() || !psi) return xf_fail(XF_EINVAL, "xf_ensemble_upload: bad arguments");
+    return ens->inst[i]->upload(psi, r, dr);
+}
+int xf_ensemble_init_pure(xf_ensemble* ens, int i) {
+    if (!ens || i < 0 || i >= (int)ens->inst.size()) return xf_fail(XF_EINVAL, "bad instance");
+    XfInst* in = ens->inst[i].get();
+    const int ne = in->t.ne;
+    std::vector<double> psi((size_t)in->t.nbeam * 16 * ne, 0.);
+    for (int b = 0; b < in->t.nbeam; ++b)
+        for (int s = 0; s < 4; ++s)
+            for (int e = 0; e < ne; ++e) psi[(((size_t)b * 4 + s) * 4 + (s < 2 ? 0 : 2)) * ne + e] = 1.;
+    return in->upload(psi.data(), in->p.R0, in->p.dr);
+}
+int xf_ensemble_download(xf_ensemble* ens, int i, double* psi, double* r, double* dr) {
+    if (!ens || i < 0 || i >= (int)ens->inst.size()) return xf_fail(XF_EINVAL, "bad instance");
+    return ens->inst[i]->download(psi, r, dr);
+}
+int xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info) {
+    if (!ens || max_steps < 0) return xf_fail(XF_EINVAL, "bad arguments");
+    double ms = 0;
+    int rc = ens_launch(ens, max_steps, nullptr, &ms);
+    if (rc) return rc;
+    int first_err = XF_OK;
+    for (size_t i = 0; i < ens->inst.size(); ++i) {
+        rc = fill_info(ens->inst[i].get(), info ? info + i : nullptr, ms);
+        if (rc && !first_err) first_err = rc;
+    }
+    return first_err;
+}
+int xf_ensemble_cells(const xf_ensemble* ens, int64_t* cells) {
+    if (!ens || !cells) return xf_fail(XF_EINVAL, "bad arguments");
+    int64_t c = 0;
+    for (auto& in : ens->inst) c += in->ncells();
+    *cells = c;
+    return XF_OK;
+}
+int xf_ensemble_launch(const xf_ensemble* ens, int* mode, int* cpt, int* block, int* grid) {
+    if (!ens || !ens->shaped) return xf_fail(XF_EINVAL, "no launch shape yet");
+    if (mode) *mode = ens->L.mode;
+    if (cpt) *cpt = ens->L.cpt;
+    if (block) *block = ens->L.block;
+    if (grid) *grid = ens->L.grid;
+    return XF_OK;
+}
+
+}  // extern "C"
+
+// access for the Phy mirror
+XfPhy*& xf_ctx_phy(xf_ctx* ctx) { return ctx->phy; }
+const xf_params& xf_ctx_params(const xf_ctx* ctx) { return ctx->p; }
+const XfTables& xf_ctx_tables(const xf_ctx* ctx) { return ctx->t; }

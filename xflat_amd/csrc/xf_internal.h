@@ -1,0 +1,117 @@
+// xf_internal.h -- shared by the host set-up (xf_setup.cpp), the Phy mirror
+// kernels (xf_phy.hip) and the fused stepper (xf_step.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/xflat.h"
+
+// Constants of include/Util.h:15-22 and include/global.h:43-46.  The
+// reference truncates pi and sqrt(2); FP64 parity needs the same values.
+namespace xfc {
+constexpr double MEV = 5.067731161976211e+15;
+constexpr double MEV_2 = 3.893793036626876e-32;
+constexpr double ERG_S = 1.053613095584080e+16;
+constexpr double EV2 = 2.568189913006477e+19;
+constexpr double SQRT2 = 1.41421356237;
+constexpr double PI = 3.14159265359;
+constexpr double GF = 1.16637E-11 * MEV_2;
+constexpr int NPR = 4;      // particles nu_e, anti-nu_e, nu_x, anti-nu_x
+}  // namespace xfc
+
+// Error reporting (thread-local last message).
+int xf_fail(int code, const char* fmt, ...);
+
+// ---- host-side set-up tables (nbm::init, phy::init, matt::init) ----------
+struct XfTables {
+    int module, nth, nph, ne, nbeam;
+    double dE;
+    std::vector<double> fv;        // [4][ne] normalised spectra (NBeam.cpp:154-208)
+    std::vector<double> hvac;      // [ne][2] (NBeam.cpp:180-184)
+    double Ev[4], L_E[4];
+    std::vector<double> cos2R, dcos2R, sinR;   // [nth] (Phy_MA.cpp:141-164, Phy_MAA.cpp:160-174)
+    std::vector<double> cosphi, sinphi;        // [nph] (Phy_MAA.cpp:177-185)
+    double dPhi;
+    double integ_coef;             // Phy_MA.cpp:271-272 / Phy_MAA.cpp:297-298
+    double mat_coeff, nbfar_coeff; // Matt.cpp:22-29
+};
+int xf_build_tables(const xf_params* p, XfTables* t);
+double xf_host_hm(const xf_params* p, const XfTables* t, double r);
+
+// ---- device descriptor of one instance for the fused stepper --------------
+// Moment layout: per source point NM = 2 (MA: J0, J1) or 4 (MAA: J0, Jx, Jy,
+// Jz) moments times 3 density components (0, 2, 3; component 1 is zero).
+struct XfRunScalars {
+    double r, dr;
+    long long total, accepted;     // counter_tot, counter (Nmr.cpp:123)
+    double last_err;
+    int cur;                       // state buffer holding psi0 (streaming)
+    int status;                    // 0 ok, XF_EBARRIER ...
+    long long steps_call, acc_call;
+    int stop_reason;
+    int pad;
+};
+
+struct XfGridBar {                 // generation barrier, one per instance
+    unsigned int count;
+    unsigned int gen;
+    unsigned int pad[62];          // own 256-byte line
+};
+
+struct XfInstDev {
+    int module, nth, nph, ne, nbeam;
+    int ns;                        // 2 (partner-reduced) or 4 species stored
+    int bpt, ntiles;               // beams per tile, tiles
+    int blk0, nblk;                // workgroups [blk0, blk0+nblk) of the launch
+    long long ncells;              // nbeam * ne
+    // tables
+    const double* x;               // cos^2 theta at R_nu [nth]
+    const double* dx;              // its bin width [nth]
+    const double* sinR;            // [nth]
+    const double* cphi;            // [nph]
+    const double* sphi;            // [nph]
+    const double* hv;              // [2][ne] H_vac components (h_r0, h_r1)
+    const double* srcw;            // [ns][ne] source weights (L/<E> fv combos)
+    const double* errw;            // [4][ne] error weights (fv of the 4 particles)
+    const float* hvf;              // float copies
+    const float* srcwf;            // srcw / its max |.| (L/<E> ~ 1e50 exceeds float)
+    const float* errwf;
+    // scalars
+    double Rv, coef, mat_coeff, nbfar_coeff, nb0, hNS;
+    double coef_f;                 // coef * source-weight scale of the float tables
+    int has_matter, renorm;
+    double eps0, ksi, max_dr, Rn;
+    // state and work buffers
+    void* psi[2];                  // [ns*4][ncells] planes, double or float
+    XfRunScalars* rs;
+    XfGridBar* bar;
+    double* part;                  // [3][nblk][NVMAX] partial sums (phases A,B,C)
+    xf_step_rec* log;              // optional per-attempt log
+    // stop predicates
+    double dump_rstep, dump_last_r;
+    int dump_itrstep;
+    int pad;
+};
+
+constexpr int XF_NVMAX = 32;       // max values per block partial
+
+// ---- launch configuration ---------------------------------------------------
+struct XfLaunch {
+    int precision;     // XF_FP64 / XF_MIXED / XF_FP32
+    int ns;            // 2 or 4
+    int mode;          // 0 resident-keep, 1 resident-recompute, 2 streaming
+    int cpt;           // 1, 2, 4
+    int block;         // threads per workgroup
+    int bpt;           // beams per tile
+    int grid;          // workgroups (all instances)
+};
+
+// Launch helpers of the fused stepper (xf_step_host.hip).
+bool xf_step_has_shape(const XfLaunch& L, int geom);
+size_t xf_step_smem_bytes(const XfLaunch& L, int geom);
+int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks);
+int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int max_steps,
+                        void* stream);
+int xf_step_func_info(const XfLaunch& L, int geom, int* vgprs, int* spill_bytes);

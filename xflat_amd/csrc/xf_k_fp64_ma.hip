@@ -1,0 +1,18 @@
+// Instantiations of the fused stepper: fp64 state/arithmetic, MA geometry.
+#include "xf_step_kernel.h"
+
+XfStepFn xf_pick_fp64_ma(int ns, int cpt, int mode) {
+    if (ns == 2 && cpt == 1 && mode == 1) return k_step<double, double, 2, XF_MA, 1, 1, 512>;
+    if (ns == 2 && cpt == 1 && mode == 2) return k_step<double, double, 2, XF_MA, 1, 2, 512>;
+    if (ns == 2 && cpt == 2 && mode == 1) return k_step<double, double, 2, XF_MA, 2, 1, 512>;
+    if (ns == 2 && cpt == 2 && mode == 2) return k_step<double, double, 2, XF_MA, 2, 2, 512>;
+    if (ns == 2 && cpt == 4 && mode == 1) return k_step<double, double, 2, XF_MA, 4, 1, 512>;
+    if (ns == 2 && cpt == 4 && mode == 2) return k_step<double, double, 2, XF_MA, 4, 2, 512>;
+    if (ns == 4 && cpt == 1 && mode == 1) return k_step<double, double, 4, XF_MA, 1, 1, 512>;
+    if (ns == 4 && cpt == 1 && mode == 2) return k_step<double, double, 4, XF_MA, 1, 2, 512>;
+    if (ns == 4 && cpt == 2 && mode == 1) return k_step<double, double, 4, XF_MA, 2, 1, 512>;
+    if (ns == 4 && cpt == 2 && mode == 2) return k_step<double, double, 4, XF_MA, 2, 2, 512>;
+    if (ns == 4 && cpt == 4 && mode == 1) return k_step<double, double, 4, XF_MA, 4, 1, 512>;
+    if (ns == 4 && cpt == 4 && mode == 2) return k_step<double, double, 4, XF_MA, 4, 2, 512>;
+    return nullptr;
+}

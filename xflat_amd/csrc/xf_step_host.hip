@@ -1,0 +1,78 @@
+// xf_step_host.hip -- launch helpers of the fused stepper: kernel choice,
+// LDS size, co-residency, cooperative launch.
+#include <hip/hip_runtime.h>
+#include "xf_internal.h"
+
+using XfStepFn = void (*)(const XfInstDev*, const int*, int);
+XfStepFn xf_pick_fp64_ma(int, int, int);
+XfStepFn xf_pick_fp64_maa(int, int, int);
+XfStepFn xf_pick_mixed_ma(int, int, int);
+XfStepFn xf_pick_mixed_maa(int, int, int);
+XfStepFn xf_pick_fp32_ma(int, int, int);
+XfStepFn xf_pick_fp32_maa(int, int, int);
+
+static XfStepFn pick(const XfLaunch& L, int geom) {
+    bool maa = geom == XF_MAA;
+    switch (L.precision) {
+        case XF_FP64: return maa ? xf_pick_fp64_maa(L.ns, L.cpt, L.mode) : xf_pick_fp64_ma(L.ns, L.cpt, L.mode);
+        case XF_FP32: return maa ? xf_pick_fp32_maa(L.ns, L.cpt, L.mode) : xf_pick_fp32_ma(L.ns, L.cpt, L.mode);
+        default: return maa ? xf_pick_mixed_maa(L.ns, L.cpt, L.mode) : xf_pick_mixed_ma(L.ns, L.cpt, L.mode);
+    }
+}
+
+bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nullptr; }
+
+size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
+    int nm = geom == XF_MAA ? 4 : 2;
+    int bsize = 3 + 3 * nm + 9 + 12;
+    int nwaves = L.block / 64;
+    return sizeof(double) * ((size_t)L.bpt * bsize + 4 * (size_t)L.block * L.cpt + 4 * (size_t)L.bpt +
+                             (size_t)nwaves * XF_NVMAX + XF_NVMAX);
+}
+
+int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {
+    XfStepFn f = pick(L, geom);
+    if (!f) return xf_fail(XF_EINVAL, "no stepper instantiated for precision %d, %d species, cpt %d, mode %d",
+                           L.precision, L.ns, L.cpt, L.mode);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return xf_fail(XF_EHIP, "hipGetDevice failed");
+    int ncu = 0;
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    size_t smem = xf_step_smem_bytes(L, geom);
+    if (smem > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    }
+    int per = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)f, L.block, smem);
+    if (e != hipSuccess) return xf_fail(XF_EHIP, "occupancy query failed: %s", hipGetErrorString(e));
+    *max_blocks = per * ncu;
+    return XF_OK;
+}
+
+int xf_step_func_info(const XfLaunch& L, int geom, int* vgprs, int* spill_bytes) {
+    XfStepFn f = pick(L, geom);
+    if (!f) return xf_fail(XF_EINVAL, "no stepper for this shape");
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, (const void*)f);
+    if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncGetAttributes: %s", hipGetErrorString(e));
+    if (vgprs) *vgprs = a.numRegs;
+    if (spill_bytes) *spill_bytes = (int)a.localSizeBytes;
+    return a.maxThreadsPerBlock;
+}
+
+int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int max_steps,
+                        void* stream) {
+    XfStepFn f = pick(L, geom);
+    if (!f) return xf_fail(XF_EINVAL, "no stepper for this shape");
+    size_t smem = xf_step_smem_bytes(L, geom);
+    if (smem > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    }
+    void* args[] = {(void*)&d_insts, (void*)&d_blk_inst, (void*)&max_steps};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)f, dim3(L.grid), dim3(L.block), args, smem, (hipStream_t)stream);
+    if (e != hipSuccess)
+        return xf_fail(XF_EHIP, "cooperative launch (grid %d x %d, %zu B LDS): %s", L.grid, L.block, smem, hipGetErrorString(e));
+    return XF_OK;
+}

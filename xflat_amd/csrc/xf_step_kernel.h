@@ -1,0 +1,858 @@
+#pragma once
+// xf_step_kernel.h -- the fused multi-angle evolution step for CDNA4 (gfx950).
+// Included by the xf_k_*.hip translation units, each instantiating a subset.
+//
+// One cooperative launch attempts up to max_steps radial steps of
+// nmr::evolutionLoop (src/Nmr.cpp:125-252) for one or many independent
+// instances.  Each instance owns a contiguous range of workgroups; a
+// workgroup owns "tiles" of whole beams (theta[, phi]) with all their energy
+// bins, one energy cell per lane slot, so per-beam energy sums stay inside
+// the workgroup.  A step is three passes over the cells:
+//
+//   A: psi1 = U(h0, dl1) psi0, psi3 = U(h0, dl0) psi0   -> moments of S(psi1), S(psi3)
+//   B: psi1 <- (U(h1, dl1) psi0 + psi1) / 2,
+//      psi4 = U(h3, dl1) psi0                             -> moments of S(psi4)
+//   C: psi3 <- U(h4, dl2) psi3, psi4 <- (psi3 + psi4) / 2,
+//      err = sum_e fv |psi4 - psi1|^2 per beam            -> moments of S(psi4), max err
+//
+// (S1..S11 of Nmr.cpp:177-206).  Between passes each workgroup reduces its
+// angular-moment partials (2 moments x 3 density components for MA,
+// Phy_MA.cpp:388-446; 4 x 3 for MAA, Phy_MAA.cpp:468-562) in a fixed order,
+// a grid barrier publishes them, and every workgroup sums all partials in the
+// same fixed order -- so all workgroups hold bit-identical moments, error,
+// accept/reject (Nmr.cpp:227) and dr (Nmr.cpp:247-249) without another sync.
+//
+// State: structure-of-arrays planes psi[species*4 + comp][cell], cell =
+// beam*NE + e, double or float.  With 2 species the nu_x / anti-nu_x columns
+// are the SU(2) partners (-conj b, conj a) of nu_e / anti-nu_e and are not
+// stored.  Modes: 0 keeps psi0 and the intermediate states in registers for
+// the whole launch (state never touches HBM), 1 keeps only psi0 in registers
+// and recomputes the intermediates in pass C, 2 streams psi0 from HBM in
+// every pass (any size) and writes the accepted candidate to the other buffer.
+//
+// Precision: FP64 mirrors NBeam::U (NBeam.cpp:360-393) operation by
+// operation.  The FP32 modes propagate increments delta = (U - I) psi in
+// FP32 with the half-angle form cos x - 1 = -2 sin^2(x/2) and hardware
+// sin/cos/rsq; moments, error sums and the decision are FP64.
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include <cstdint>
+#include <cstdio>
+
+#include "xf_internal.h"
+
+using XfStepFn = void (*)(const XfInstDev*, const int*, int);
+
+namespace {
+
+constexpr double INV_2PI_D = 0.15915494309189535;
+constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;   // 2 s at 100 MHz
+
+template <class T> struct Sp { T ar, ai, br, bi; };
+
+// ---------------------------------------------------------------- FP64 path
+struct PropD { double m0, m1, m2, m3; };
+
+__device__ __forceinline__ double lam_d(double h0, double h1, double h2) {
+    return sqrt(h0 * h0 + (h1 * h1 + h2 * h2));
+}
+__device__ __forceinline__ void prop_d(double h0, double h1, double h2, double lam, double dl, PropD& P) {
+    double ldr = lam * dl;
+    double s, c;
+    sincos(ldr, &s, &c);
+    double sc = s / lam;
+    P.m0 = c;
+    P.m1 = -h0 * sc;
+    P.m2 = h2 * sc;
+    P.m3 = -h1 * sc;
+}
+__device__ __forceinline__ Sp<double> apply_d(const PropD& P, const Sp<double>& a) {
+    Sp<double> o;
+    o.ar = (P.m0 * a.ar - P.m1 * a.ai) + (P.m2 * a.br - P.m3 * a.bi);
+    o.ai = (P.m0 * a.ai + P.m1 * a.ar) + (P.m2 * a.bi + P.m3 * a.br);
+    o.br = (-P.m2 * a.ar - P.m3 * a.ai) + (P.m0 * a.br + P.m1 * a.bi);
+    o.bi = (-P.m2 * a.ai + P.m3 * a.ar) + (P.m0 * a.bi - P.m1 * a.br);
+    return o;
+}
+
+// ---------------------------------------------------------------- FP32 path
+// (U - I) = [[cm1 + i m1, m2 + i m3], [-m2 + i m3, cm1 - i m1]]
+struct PropF { float cm1, m1, m2, m3; };
+
+__device__ __forceinline__ void prop_f(float h0, float h1, float h2, float dl, PropF& P) {
+    float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
+    float rl = lam2 > 0.f ? __builtin_amdgcn_rsqf(lam2) : 0.f;
+    float lam = lam2 * rl;
+    float x = lam * dl * (float)(0.5 * INV_2PI_D);         // half angle, revolutions
+    x = x - __builtin_rintf(x);
+    float sh = __builtin_amdgcn_sinf(x);
+    float ch = __builtin_amdgcn_cosf(x);
+    float s = 2.f * sh * ch;                               // sin(lam dl)
+    P.cm1 = -2.f * sh * sh;                                // cos(lam dl) - 1
+    float sc = s * rl;
+    P.m1 = -h0 * sc;
+    P.m2 = h2 * sc;
+    P.m3 = -h1 * sc;
+}
+__device__ __forceinline__ Sp<float> delta_f(const PropF& P, const Sp<float>& a) {
+    Sp<float> o;
+    o.ar = (P.cm1 * a.ar - P.m1 * a.ai) + (P.m2 * a.br - P.m3 * a.bi);
+    o.ai = (P.cm1 * a.ai + P.m1 * a.ar) + (P.m2 * a.bi + P.m3 * a.br);
+    o.br = (-P.m2 * a.ar - P.m3 * a.ai) + (P.cm1 * a.br + P.m1 * a.bi);
+    o.bi = (-P.m2 * a.ai + P.m3 * a.ar) + (P.cm1 * a.bi - P.m1 * a.br);
+    return o;
+}
+template <class T> __device__ __forceinline__ Sp<T> sp_add(const Sp<T>& a, const Sp<T>& b) {
+    return Sp<T>{a.ar + b.ar, a.ai + b.ai, a.br + b.br, a.bi + b.bi};
+}
+// (new + old) * .5, the averaging of NBeam.cpp:507-510 and :730-733
+template <class T> __device__ __forceinline__ Sp<T> sp_avg(const Sp<T>& nw, const Sp<T>& old) {
+    return Sp<T>{(nw.ar + old.ar) * T(.5), (nw.ai + old.ai) * T(.5), (nw.br + old.br) * T(.5), (nw.bi + old.bi) * T(.5)};
+}
+__device__ __forceinline__ Sp<float> sp_f(const Sp<double>& a) {
+    return Sp<float>{(float)a.ar, (float)a.ai, (float)a.br, (float)a.bi};
+}
+__device__ __forceinline__ Sp<float> sp_f(const Sp<float>& a) { return a; }
+__device__ __forceinline__ Sp<double> sp_d(const Sp<float>& a) {
+    return Sp<double>{(double)a.ar, (double)a.ai, (double)a.br, (double)a.bi};
+}
+__device__ __forceinline__ Sp<double> sp_d(const Sp<double>& a) { return a; }
+
+// density matrix components 0, 2, 3 (NBeam.cpp:314-322)
+template <class T> __device__ __forceinline__ void dens(const Sp<T>& a, T& d0, T& d2, T& d3) {
+    d0 = T(.5) * ((a.ar * a.ar + a.ai * a.ai) - (a.br * a.br + a.bi * a.bi));
+    d2 = (a.ar * a.br) - (a.ai * (-a.bi));
+    d3 = (a.ar * (-a.bi)) + (a.ai * a.br);
+}
+template <class T> __device__ __forceinline__ T dnorm2(const Sp<T>& x, const Sp<T>& y) {
+    T dar = x.ar - y.ar, dai = x.ai - y.ai, dbr = x.br - y.br, dbi = x.bi - y.bi;
+    return (dar * dar + dai * dai) + (dbr * dbr + dbi * dbi);
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double ld_agent(const double* p) {
+    unsigned long long u = __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)u);
+}
+__device__ __forceinline__ void st_agent(double* p, double v) {
+    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Generation barrier over the nblk workgroups of one instance.  The wait is
+// bounded (2 s of the 100 MHz real-time counter); on timeout, or when another
+// workgroup has flagged an error, the status word is set and false returned,
+// so every workgroup leaves its step loop and the launch ends.
+__device__ __forceinline__ bool grid_sync(XfGridBar* bar, unsigned nblk, unsigned& gen, int* status, bool fence_all, int* s_flag) {
+    if (fence_all) __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        unsigned old = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == nblk - 1) {
+            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
+                if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
+                    __hip_atomic_store(status, (int)XF_EBARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        gen += 1u;
+        *s_flag = ok;
+    }
+    __syncthreads();
+    if (fence_all) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *s_flag != 0;
+}
+
+// matt::getHm (Matt.cpp:31-64)
+__device__ __forceinline__ double hm_at(const XfInstDev& I, double r) {
+    if (!I.has_matter) return 0.;
+    double R = 10. / r;
+    double far = I.nbfar_coeff * (R * R * R);
+    double near = I.nb0 * exp((I.Rv - r) / I.hNS);
+    return I.mat_coeff * (far + near) * .5;
+}
+
+// Per-beam record in LDS (doubles).
+template <int GEOM> struct BRec {
+    static constexpr int NM = (GEOM == XF_MAA) ? 4 : 2;
+    static constexpr int DL = 0;                  // dl[3]
+    static constexpr int W = 3;                   // W[3 pts][NM] moment weights
+    static constexpr int G = W + 3 * NM;          // G[3 pts][3] hvv geometry
+    static constexpr int H = G + 9;               // Hb[4][3]: h0, h1, h3, h4 (+matter)
+    static constexpr int SIZE = H + 12;
+};
+
+// Geometry of one beam at r, r+dr/2, r+dr: angle bins (Phy_MA.cpp:172-196,
+// Phy_MAA.cpp:194-210) and path lengths (Phy_MA.cpp:218-227, Phy_MAA.cpp:232-241).
+template <int GEOM>
+__device__ void beam_geometry(const XfInstDev& I, int b, double r, double rdr2, double rdr, double dr,
+                              double dr2, double* rec) {
+    using R = BRec<GEOM>;
+    int th = (GEOM == XF_MAA) ? b / I.nph : b;
+    double x = I.x[th], dx = I.dx[th];
+    double rp[3] = {r, rdr2, rdr};
+    double cs[3];
+#pragma unroll
+    for (int pt = 0; pt < 3; ++pt) {
+        if (GEOM == XF_MAA) {
+            int ph = b - th * I.nph;
+            double ra = I.Rv / rp[pt];
+            double r2 = ra * ra;
+            double c = sqrt(1. - r2 * (1. - x));
+            double s = ra * I.sinR[th];
+            double dc = .5 * (1. / c) * r2 * dx;
+            double cp = I.cphi[ph], spp = I.sphi[ph];
+            rec[R::W + pt * 4 + 0] = dc;
+            rec[R::W + pt * 4 + 1] = cp * s * dc;
+            rec[R::W + pt * 4 + 2] = spp * s * dc;
+            rec[R::W + pt * 4 + 3] = dc * c;
+            rec[R::G + pt * 3 + 0] = s * cp;
+            rec[R::G + pt * 3 + 1] = s * spp;
+            rec[R::G + pt * 3 + 2] = c;
+            cs[pt] = c;
+        } else {
+            double r2 = I.Rv / rp[pt];
+            r2 *= r2;
+            double c = sqrt(1. - r2 * (1. - x));
+            double dc = .5 * (1. / c) * r2 * dx;
+            rec[R::W + pt * 2 + 0] = dc;
+            rec[R::W + pt * 2 + 1] = dc * c;
+            rec[R::G + pt * 3 + 0] = c;
+            rec[R::G + pt * 3 + 1] = 0.;
+            rec[R::G + pt * 3 + 2] = 0.;
+            cs[pt] = c;
+        }
+    }
+    if (GEOM == XF_MAA) {
+        rec[R::DL + 0] = dr2 / ((cs[0] + cs[1]) * .5);
+        rec[R::DL + 1] = dr / ((cs[0] + cs[2]) * .5);
+        rec[R::DL + 2] = dr2 / ((cs[1] + cs[2]) * .5);
+    } else {
+        rec[R::DL + 0] = dr2 / (cs[0]);
+        rec[R::DL + 1] = dr / (cs[0]);
+        rec[R::DL + 2] = dr2 / (cs[1]);
+    }
+}
+
+// H_vv of one beam at point pt from the moments M[m*3+z] (z: density comps
+// 0,2,3): Phy_MA.cpp:372-377 / Phy_MAA.cpp:445-461; plus the matter term on
+// component 0 (NBeam.cpp:604).
+template <int GEOM>
+__device__ __forceinline__ void beam_h(const double* rec, int pt, const double* M, double coef, double hm, double* Hb) {
+    using R = BRec<GEOM>;
+    const double* G = rec + R::G + pt * 3;
+    double hv[3];
+#pragma unroll
+    for (int z = 0; z < 3; ++z) {
+        if (GEOM == XF_MAA)
+            hv[z] = (M[z] - (G[0] * M[3 + z] + G[1] * M[6 + z] + G[2] * M[9 + z])) * coef;
+        else
+            hv[z] = (M[z] - (M[3 + z] * G[0])) * coef;
+    }
+    Hb[0] = hv[0] + hm;
+    Hb[1] = hv[1];
+    Hb[2] = hv[2];
+}
+
+// Per-cell data of one thread.
+template <class ST, class CT, int NS, int CPT> struct Cells {
+    static constexpr bool F64 = std::is_same<CT, double>::value;
+    int bl[CPT];         // beam within the tile, -1 if the slot is empty
+    int e[CPT];
+    Sp<ST> p0[CPT][NS];  // psi0 (resident modes)
+    Sp<CT> k1[CPT][NS];  // psi1 (FP64) / its increment (FP32)
+    Sp<CT> k3[CPT][NS];  // psi3
+    Sp<CT> k4[CPT][NS];  // psi4
+};
+
+template <class ST>
+__device__ __forceinline__ Sp<ST> ld_sp(const ST* __restrict__ base, long long ncells, int s, long long cell) {
+    const ST* p = base + (long long)(s * 4) * ncells + cell;
+    return Sp<ST>{p[0], p[ncells], p[2 * ncells], p[3 * ncells]};
+}
+template <class ST>
+__device__ __forceinline__ void st_sp(ST* __restrict__ base, long long ncells, int s, long long cell, const Sp<double>& v) {
+    ST* p = base + (long long)(s * 4) * ncells + cell;
+    p[0] = (ST)v.ar;
+    p[ncells] = (ST)v.ai;
+    p[2 * ncells] = (ST)v.br;
+    p[3 * ncells] = (ST)v.bi;
+}
+
+// Hamiltonian of species s at energy e for beam H record Hb (NBeam.cpp:602-625):
+// nu: (hv0 + Hb0, hv1 + Hb1, Hb2); anti-nu: (hv0 - Hb0, hv1 - Hb1, Hb2).
+__device__ __forceinline__ void cell_h(const XfInstDev& I, int e, int s, const double* Hb, double& h0, double& h1,
+                                       double& h2) {
+    double hv0 = I.hv[e], hv1 = I.hv[I.ne + e];
+    if (s & 1) {
+        h0 = hv0 + (-Hb[0]);
+        h1 = hv1 + (-Hb[1]);
+    } else {
+        h0 = hv0 + Hb[0];
+        h1 = hv1 + Hb[1];
+    }
+    h2 = Hb[2];
+}
+
+// Source contribution of one cell to the angular integral, X(z) for density
+// components 0, 2, 3 (upd_nu_coef, NBeam.h:47-53, with L/<E> folded into the
+// per-energy source weights).
+template <class CT>
+__device__ __forceinline__ void src_add(const XfInstDev& I, int e, int s, const Sp<CT>& v, CT& X0, CT& X2, CT& X3) {
+    CT d0, d2, d3;
+    dens<CT>(v, d0, d2, d3);
+    CT w;
+    if constexpr (std::is_same<CT, double>::value) w = I.srcw[s * I.ne + e];
+    else w = I.srcwf[s * I.ne + e];
+    if (s & 1) { X0 -= w * d0; X2 -= w * d2; }
+    else       { X0 += w * d0; X2 += w * d2; }
+    X3 += w * d3;
+}
+
+template <int NM>
+__device__ __forceinline__ void acc_moments(double* acc, const double* W, double X0, double X2, double X3) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        acc[m * 3 + 0] += X0 * W[m];
+        acc[m * 3 + 1] += X2 * W[m];
+        acc[m * 3 + 2] += X3 * W[m];
+    }
+}
+
+
+// psi0 + increment, formed in FP64; with renorm the unitarity of the state
+// |a|^2 + |b|^2 = 1 is restored (FP32 modes only).
+__device__ __forceinline__ Sp<double> candidate(const Sp<double>& p0, const Sp<float>& df, int renorm) {
+    Sp<double> nd{p0.ar + (double)df.ar, p0.ai + (double)df.ai, p0.br + (double)df.br, p0.bi + (double)df.bi};
+    if (renorm) {
+        double n2 = (nd.ar * nd.ar + nd.ai * nd.ai) + (nd.br * nd.br + nd.bi * nd.bi);
+        double sc = 1.0 / sqrt(n2);
+        nd.ar *= sc; nd.ai *= sc; nd.br *= sc; nd.bi *= sc;
+    }
+    return nd;
+}
+
+// Block reduction of NV sums and NMAX maxima -> s_tot; thread 0 publishes
+// them as this workgroup's partial of phase `ph` and joins the barrier;
+// afterwards every workgroup sums all partials in the same fixed order.
+template <int NV, int NMAX>
+__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red, double* s_tot, const XfInstDev& I, int lb, int ph,
+                               unsigned& gen, int* status, bool fence_all, int* s_flag) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int nblk = I.nblk;
+#pragma unroll
+    for (int v = 0; v < NV + NMAX; ++v) {
+        double x = (v < NV) ? wave_sum(acc[v]) : wave_max(acc[v]);
+        if (lane == 0) s_red[wave * XF_NVMAX + v] = x;
+    }
+    __syncthreads();
+    if (tid < NV + NMAX) {
+        double x = s_red[tid];
+        for (int w = 1; w < nwaves; ++w)
+            x = (tid < NV) ? x + s_red[w * XF_NVMAX + tid] : fmax(x, s_red[w * XF_NVMAX + tid]);
+        s_tot[tid] = x;
+    }
+    __syncthreads();
+    double* part = I.part + ((long long)ph * nblk + lb) * XF_NVMAX;
+    if (tid == 0)
+        for (int v = 0; v < NV + NMAX; ++v) st_agent(part + v, s_tot[v]);
+    bool ok = grid_sync(I.bar, (unsigned)nblk, gen, status, fence_all, s_flag);
+    const double* pall = I.part + (long long)ph * nblk * XF_NVMAX;
+    for (int v = wave; v < NV + NMAX; v += nwaves) {
+        const bool is_max = v >= NV;
+        double x = 0.;
+        for (int j = lane; j < nblk; j += 64) {
+            double y = ld_agent(pall + (long long)j * XF_NVMAX + v);
+            x = is_max ? fmax(x, y) : x + y;
+        }
+        x = is_max ? wave_max(x) : wave_sum(x);
+        if (lane == 0) s_tot[v] = x;
+    }
+    __syncthreads();
+    return ok;
+}
+
+}  // namespace
+
+// ============================================================================
+// The kernel.
+// ============================================================================
+template <class ST, class CT, int NS, int GEOM, int CPT, int MODE, int MAXB>
+__global__ void __launch_bounds__(MAXB)
+k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, int max_steps) {
+    constexpr bool F64 = std::is_same<CT, double>::value;
+    using R = BRec<GEOM>;
+    constexpr int NM = R::NM;
+    constexpr int NMV = NM * 3;
+    static_assert(2 * NMV + 1 <= XF_NVMAX, "partials");
+
+    const XfInstDev& I = insts[blk_inst[blockIdx.x]];
+    const int B = blockDim.x;
+    if ((B & 63) != 0 || B > MAXB) {
+        // whole waves only; every workgroup leaves before any barrier
+        if (threadIdx.x == 0) __hip_atomic_store(&I.rs->status, (int)XF_EINVAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = B >> 6;
+    const int lb = blockIdx.x - I.blk0;
+    const int nblk = I.nblk;
+    const int ne = I.ne, bpt = I.bpt, ntiles = I.ntiles;
+    const long long ncells = I.ncells;
+    XfRunScalars* rs = I.rs;
+
+    extern __shared__ double smem[];
+    double* s_beam = smem;                                   // [bpt][R::SIZE]
+    double* s_errb = s_beam + bpt * R::SIZE;                 // [4][B*CPT]
+    double* s_errs = s_errb + 4 * B * CPT;                   // [bpt][4]
+    double* s_red = s_errs + 4 * bpt;                        // [nwaves][XF_NVMAX]
+    double* s_tot = s_red + nwaves * XF_NVMAX;               // [XF_NVMAX]
+    __shared__ int s_flag;
+    // angular moments, uniform over the workgroup: psi0, psi1, psi3, psi4, candidate
+    __shared__ double s_M0[12], s_M1[12], s_M3[12], s_M4[12], s_Mn[12];
+
+    // uniform step state
+    double r = rs->r, dr = rs->dr;
+    long long total = rs->total, accepted = rs->accepted;
+    int cur = rs->cur;
+    double last_err = rs->last_err;
+    unsigned gen = __hip_atomic_load(&I.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int* status = &rs->status;
+
+    Cells<ST, CT, NS, CPT> C;
+    double* M0 = s_M0;
+    double* M1 = s_M1;
+    double* M3 = s_M3;
+    double* M4 = s_M4;
+    double* Mn = s_Mn;
+
+    // ---- tile helpers -------------------------------------------------------
+    auto tile_cells = [&](int t) __attribute__((always_inline)) {
+        int beam0 = t * bpt;
+        int nbt = max(0, min(bpt, I.nbeam - beam0));
+        int ncl = nbt * ne;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            int i = tid + k * B;
+            if (t < ntiles && i < ncl) {
+                int b = i / ne;
+                C.bl[k] = b;
+                C.e[k] = i - b * ne;
+            } else {
+                C.bl[k] = -1;
+                C.e[k] = 0;
+            }
+        }
+        return nbt;
+    };
+    auto cell_index = [&](int t, int k) __attribute__((always_inline)) -> long long {
+        return (long long)t * bpt * ne + (long long)C.bl[k] * ne + C.e[k];
+    };
+    auto load_p0 = [&](int t, int buf) __attribute__((always_inline)) {
+        const ST* base = (const ST*)I.psi[buf];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            if (C.bl[k] >= 0) {
+                long long c = cell_index(t, k);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) C.p0[k][s] = ld_sp<ST>(base, ncells, s, c);
+            }
+        }
+    };
+    // geometry of the tile's beams at the current step; Hb filled separately
+    auto tile_geometry = [&](int t, int nbt, double rr, double rdr2, double rdr, double ddr, double dr2) __attribute__((always_inline)) {
+        for (int b = tid; b < nbt; b += B) beam_geometry<GEOM>(I, t * bpt + b, rr, rdr2, rdr, ddr, dr2, s_beam + b * R::SIZE);
+    };
+    auto tile_h = [&](int nbt, int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
+        const double coef = F64 ? I.coef : I.coef_f;   // float source weights are pre-scaled
+        for (int b = tid; b < nbt; b += B) beam_h<GEOM>(s_beam + b * R::SIZE, pt, M, coef, hm, s_beam + b * R::SIZE + R::H + slot * 3);
+    };
+
+    const int my_tile0 = lb;                 // resident: exactly this tile
+    int nbt_res = 0;
+    if (MODE != 2) {
+        nbt_res = tile_cells(my_tile0);
+        load_p0(my_tile0, cur);
+    }
+
+    bool ok = true;
+    // ---- moments of psi0 at r (the cached energy sums of Nmr.cpp:117-121)
+    {
+        double acc[XF_NVMAX];
+#pragma unroll
+        for (int v = 0; v < NMV; ++v) acc[v] = 0.;
+        double dr2 = .5 * dr, rdr = r + dr, rdr2 = r + dr2;
+        for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
+            int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
+            if (MODE == 2) load_p0(t, cur);
+            __syncthreads();
+            tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                if (C.bl[k] < 0) continue;
+                const double* rec = s_beam + C.bl[k] * R::SIZE;
+                CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    if constexpr (F64) src_add<CT>(I, C.e[k], s, sp_d(C.p0[k][s]), X0, X2, X3);
+                    else src_add<CT>(I, C.e[k], s, sp_f(C.p0[k][s]), X0, X2, X3);
+                }
+                acc_moments<NM>(acc, rec + R::W + 0 * NM, (double)X0, (double)X2, (double)X3);
+            }
+        }
+        ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, 2, gen, status, false, &s_flag);
+        if (tid < NMV) M0[tid] = s_tot[tid];
+        __syncthreads();
+    }
+
+    long long steps_call = 0, acc_call = 0;
+    int stop_reason = 0;
+    while (ok) {
+        __syncthreads();
+        if (!(steps_call < max_steps)) { stop_reason = 0; break; }
+        if (!(r < I.Rn)) { stop_reason = 1; break; }
+        const double dr2 = .5 * dr;
+        const double rdr = r + dr;
+        const double rdr2 = r + dr2;
+        const double hm_r = hm_at(I, r), hm_rdr2 = hm_at(I, rdr2), hm_rdr = hm_at(I, rdr);
+
+        // =================================================== pass A (S1, S2, S6)
+        {
+            double acc[XF_NVMAX];
+#pragma unroll
+            for (int v = 0; v < 2 * NMV; ++v) acc[v] = 0.;
+            for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
+                int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
+                if (MODE == 2) load_p0(t, cur);
+                __syncthreads();
+                tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+                __syncthreads();
+                tile_h(nbt, 0, 0, M0, hm_r);
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < CPT; ++k) {
+                    if (C.bl[k] < 0) continue;
+                    const double* rec = s_beam + C.bl[k] * R::SIZE;
+                    const int e = C.e[k];
+                    CT X0 = CT(0), X2 = CT(0), X3 = CT(0), Y0 = CT(0), Y2 = CT(0), Y3 = CT(0);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        double h0, h1, h2;
+                        cell_h(I, e, s, rec + R::H + 0, h0, h1, h2);
+                        if constexpr (F64) {
+                            double lam = lam_d(h0, h1, h2);
+                            PropD P1, P0;
+                            prop_d(h0, h1, h2, lam, rec[R::DL + 1], P1);
+                            prop_d(h0, h1, h2, lam, rec[R::DL + 0], P0);
+                            Sp<double> a = sp_d(C.p0[k][s]);
+                            Sp<double> v1 = apply_d(P1, a), v3 = apply_d(P0, a);
+                            if (MODE == 0) { C.k1[k][s] = v1; C.k3[k][s] = v3; }
+                            src_add<CT>(I, e, s, v1, X0, X2, X3);
+                            src_add<CT>(I, e, s, v3, Y0, Y2, Y3);
+                        } else {
+                            PropF P1, P0;
+                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], P1);
+                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 0], P0);
+                            Sp<float> a = sp_f(C.p0[k][s]);
+                            Sp<float> d1 = delta_f(P1, a), d3 = delta_f(P0, a);
+                            if (MODE == 0) { C.k1[k][s] = d1; C.k3[k][s] = d3; }
+                            src_add<CT>(I, e, s, sp_add(a, d1), X0, X2, X3);
+                            src_add<CT>(I, e, s, sp_add(a, d3), Y0, Y2, Y3);
+                        }
+                    }
+                    acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);        // h1 at r+dr
+                    acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
+                }
+            }
+            ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, I, lb, 0, gen, status, false, &s_flag);
+            if (tid < NMV) { M1[tid] = s_tot[tid]; M3[tid] = s_tot[NMV + tid]; }
+            __syncthreads();
+            if (!ok) break;
+        }
+
+        // =================================================== pass B (S3, S4/S5, S7, S8)
+        {
+            double acc[XF_NVMAX];
+#pragma unroll
+            for (int v = 0; v < NMV; ++v) acc[v] = 0.;
+            for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
+                int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
+                if (MODE == 2) {
+                    load_p0(t, cur);
+                    __syncthreads();
+                    tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+                }
+                __syncthreads();
+                tile_h(nbt, 1, 2, M1, hm_rdr);
+                tile_h(nbt, 2, 1, M3, hm_rdr2);
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < CPT; ++k) {
+                    if (C.bl[k] < 0) continue;
+                    const double* rec = s_beam + C.bl[k] * R::SIZE;
+                    const int e = C.e[k];
+                    CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        double h0, h1, h2, g0, g1, g2;
+                        cell_h(I, e, s, rec + R::H + 3, h0, h1, h2);      // h1 (+ Hm(r+dr))
+                        cell_h(I, e, s, rec + R::H + 6, g0, g1, g2);      // h3 (+ Hm(r+dr/2))
+                        if constexpr (F64) {
+                            Sp<double> a = sp_d(C.p0[k][s]);
+                            PropD Pb, P4;
+                            prop_d(h0, h1, h2, lam_d(h0, h1, h2), rec[R::DL + 1], Pb);
+                            prop_d(g0, g1, g2, lam_d(g0, g1, g2), rec[R::DL + 1], P4);
+                            Sp<double> b = apply_d(Pb, a);
+                            if (MODE == 0) C.k1[k][s] = sp_avg(b, C.k1[k][s]);
+                            Sp<double> v4 = apply_d(P4, a);
+                            if (MODE == 0) C.k4[k][s] = v4;
+                            src_add<CT>(I, e, s, v4, X0, X2, X3);
+                        } else {
+                            Sp<float> a = sp_f(C.p0[k][s]);
+                            PropF Pb, P4;
+                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], Pb);
+                            prop_f((float)g0, (float)g1, (float)g2, (float)rec[R::DL + 1], P4);
+                            Sp<float> db = delta_f(Pb, a);
+                            if (MODE == 0) C.k1[k][s] = sp_avg(db, C.k1[k][s]);
+                            Sp<float> d4 = delta_f(P4, a);
+                            if (MODE == 0) C.k4[k][s] = d4;
+                            src_add<CT>(I, e, s, sp_add(a, d4), X0, X2, X3);
+                        }
+                    }
+                    acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
+                }
+            }
+            ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, 1, gen, status, false, &s_flag);
+            if (tid < NMV) M4[tid] = s_tot[tid];
+            __syncthreads();
+            if (!ok) break;
+        }
+
+        // =================================================== pass C (S9, S10/S11, error)
+        double errmax;
+        {
+            double acc[XF_NVMAX];
+#pragma unroll
+            for (int v = 0; v < NMV + 1; ++v) acc[v] = 0.;
+            double blk_err = 0.;
+            for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
+                int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
+                if (MODE == 2) {
+                    load_p0(t, cur);
+                    __syncthreads();
+                    tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+                    __syncthreads();
+                    tile_h(nbt, 0, 0, M0, hm_r);
+                    tile_h(nbt, 1, 2, M1, hm_rdr);
+                    tile_h(nbt, 2, 1, M3, hm_rdr2);
+                }
+                __syncthreads();
+                tile_h(nbt, 3, 2, M4, hm_rdr);
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < CPT; ++k) {
+                    const int slot = k * B + tid;
+                    if (C.bl[k] < 0) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) s_errb[q * B * CPT + slot] = 0.;
+                        continue;
+                    }
+                    const double* rec = s_beam + C.bl[k] * R::SIZE;
+                    const int e = C.e[k];
+                    CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        double dsq;
+                        double h0, h1, h2;
+                        cell_h(I, e, s, rec + R::H + 9, h0, h1, h2);     // h4 (+ Hm(r+dr))
+                        if constexpr (F64) {
+                            Sp<double> a = sp_d(C.p0[k][s]);
+                            Sp<double> q1, q3, q4;
+                            if (MODE == 0) {
+                                q1 = C.k1[k][s]; q3 = C.k3[k][s]; q4 = C.k4[k][s];
+                            } else {
+                                double x0, x1, x2;
+                                cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);
+                                double lam = lam_d(x0, x1, x2);
+                                PropD P1, P0, Pb, P4;
+                                prop_d(x0, x1, x2, lam, rec[R::DL + 1], P1);
+                                prop_d(x0, x1, x2, lam, rec[R::DL + 0], P0);
+                                q1 = apply_d(P1, a);
+                                q3 = apply_d(P0, a);
+                                cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);
+                                prop_d(x0, x1, x2, lam_d(x0, x1, x2), rec[R::DL + 1], Pb);
+                                q1 = sp_avg(apply_d(Pb, a), q1);
+                                cell_h(I, e, s, rec + R::H + 6, x0, x1, x2);
+                                prop_d(x0, x1, x2, lam_d(x0, x1, x2), rec[R::DL + 1], P4);
+                                q4 = apply_d(P4, a);
+                            }
+                            PropD P2;
+                            prop_d(h0, h1, h2, lam_d(h0, h1, h2), rec[R::DL + 2], P2);
+                            Sp<double> q3n = apply_d(P2, q3);
+                            Sp<double> fin = sp_avg(q3n, q4);
+                            dsq = dnorm2<double>(fin, q1);
+                            src_add<CT>(I, e, s, fin, X0, X2, X3);
+                            if (MODE != 2) C.k4[k][s] = fin;
+                            if (MODE == 2) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), fin);
+                        } else {
+                            Sp<float> a = sp_f(C.p0[k][s]);
+                            Sp<float> d1, d3, d4;
+                            if (MODE == 0) {
+                                d1 = C.k1[k][s]; d3 = C.k3[k][s]; d4 = C.k4[k][s];
+                            } else {
+                                double x0, x1, x2;
+                                PropF P1, P0, Pb, P4;
+                                cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);
+                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P1);
+                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 0], P0);
+                                d1 = delta_f(P1, a);
+                                d3 = delta_f(P0, a);
+                                cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);
+                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], Pb);
+                                d1 = sp_avg(delta_f(Pb, a), d1);
+                                cell_h(I, e, s, rec + R::H + 6, x0, x1, x2);
+                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P4);
+                                d4 = delta_f(P4, a);
+                            }
+                            PropF P2;
+                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 2], P2);
+                            Sp<float> d3n = sp_add(d3, delta_f(P2, sp_add(a, d3)));
+                            Sp<float> df = sp_avg(d3n, d4);
+                            dsq = (double)dnorm2<float>(df, d1);
+                            Sp<double> nd = candidate(sp_d(C.p0[k][s]), df, I.renorm);
+                            Sp<float> vf = sp_f(nd);
+                            src_add<CT>(I, e, s, vf, X0, X2, X3);
+                            if constexpr (std::is_same<ST, float>::value) {
+                                if (MODE != 2) C.k4[k][s] = vf;        // the candidate state itself
+                            } else {
+                                if (MODE != 2) C.k4[k][s] = df;        // increment; re-formed on accept
+                            }
+                            if (MODE == 2) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), nd);
+                        }
+                        // error weights of the 4 particles (NBeam.cpp:922): with 2 stored
+                        // species nu_x / anti-nu_x share |dpsi|^2 with nu_e / anti-nu_e
+                        if (NS == 2) {
+                            s_errb[s * B * CPT + slot] = I.errw[s * ne + e] * dsq;
+                            s_errb[(s + 2) * B * CPT + slot] = I.errw[(s + 2) * ne + e] * dsq;
+                        } else {
+                            s_errb[s * B * CPT + slot] = I.errw[s * ne + e] * dsq;
+                        }
+                    }
+                    acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // new psi0 at r+dr
+                }
+                // per-beam energy sums of the error (one wave per beam, fixed order)
+                __syncthreads();
+                for (int b = wave; b < nbt; b += nwaves) {
+                    double sq[4] = {0., 0., 0., 0.};
+                    for (int c = b * ne + lane; c < (b + 1) * ne; c += 64)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) sq[q] += s_errb[q * B * CPT + c];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        double w = wave_sum(sq[q]);
+                        if (lane == 0) s_errs[b * 4 + q] = w;
+                    }
+                }
+                __syncthreads();
+                for (int b = tid; b < nbt; b += B) {
+                    double m = fmax(fmax(s_errs[b * 4 + 0], s_errs[b * 4 + 1]), fmax(s_errs[b * 4 + 2], s_errs[b * 4 + 3]));
+                    blk_err = fmax(blk_err, m);
+                }
+            }
+            acc[NMV] = blk_err;
+            ok = reduce_publish<NMV, 1>(acc, s_red, s_tot, I, lb, 2, gen, status, MODE == 2, &s_flag);
+            if (tid < NMV) Mn[tid] = s_tot[tid];
+            errmax = s_tot[NMV];
+            __syncthreads();
+            if (!ok) break;
+        }
+
+        // =================================================== decision (Nmr.cpp:195-249)
+        const double err = sqrt(errmax);
+        const bool acc_ok = err < I.eps0;
+        if (lb == 0 && tid == 0 && I.log) {
+            xf_step_rec rec;
+            rec.r = r; rec.dr = dr; rec.err = err; rec.accepted = acc_ok ? 1 : 0; rec.pad = 0;
+            I.log[steps_call] = rec;
+        }
+        const long long it = total;
+        if (acc_ok) {
+            r += dr;
+            if (MODE == 2) {
+                cur ^= 1;
+            } else {
+#pragma unroll
+                for (int k = 0; k < CPT; ++k)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if constexpr (F64 || std::is_same<ST, float>::value) {
+                            C.p0[k][s] = Sp<ST>{(ST)C.k4[k][s].ar, (ST)C.k4[k][s].ai, (ST)C.k4[k][s].br, (ST)C.k4[k][s].bi};
+                        } else {
+                            Sp<double> nd = candidate(sp_d(C.p0[k][s]), C.k4[k][s], I.renorm);
+                            C.p0[k][s] = Sp<ST>{(ST)nd.ar, (ST)nd.ai, (ST)nd.br, (ST)nd.bi};
+                        }
+                    }
+            }
+            if (tid < NMV) M0[tid] = Mn[tid];
+            ++accepted;
+            ++acc_call;
+        }
+        ++total;
+        ++steps_call;
+        last_err = err;
+        dr *= I.ksi * sqrt(I.eps0 / err);
+        dr = (dr > I.max_dr) ? I.max_dr : dr;
+        dr = (r + dr > I.Rn) ? I.Rn - r : dr;
+        if (acc_ok) {
+            bool due = false;
+            if (I.dump_rstep > 0. && (r == I.Rn || r - I.dump_last_r > I.dump_rstep)) due = true;
+            if (I.dump_itrstep > 0 && (it % I.dump_itrstep) == 0) due = true;
+            if (due) { stop_reason = 2; break; }
+        }
+    }
+    if (!ok) stop_reason = 3;
+
+    // ---- write back
+    if (MODE != 2 && my_tile0 < ntiles) {
+        ST* base = (ST*)I.psi[cur];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            if (C.bl[k] < 0) continue;
+            long long c = cell_index(my_tile0, k);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) st_sp<ST>(base, ncells, s, c, sp_d(C.p0[k][s]));
+        }
+    }
+    if (lb == 0 && tid == 0) {
+        rs->r = r;
+        rs->dr = dr;
+        rs->total = total;
+        rs->accepted = accepted;
+        rs->cur = cur;
+        rs->last_err = last_err;
+        rs->steps_call = steps_call;
+        rs->acc_call = acc_call;
+        rs->stop_reason = stop_reason;
+    }
+}
+
