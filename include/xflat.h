@@ -205,6 +205,14 @@ int  xf_set_launch(xf_ctx* ctx, int mode, int cpt, int block, int bpt);
 /* Registers, scratch bytes per lane, co-resident workgroups and LDS bytes of
  * the launch shape in use. */
 int  xf_launch_info(const xf_ctx* ctx, int* vgprs, int* scratch, int* max_blocks, int* lds_bytes);
+/* Profiling: per-phase real-time stamps (100 MHz counter) of workgroup 0
+ * for the first `steps` steps of the next xf_run_steps (16 slots per step:
+ * [0] step start; [1..4] pass A: cells done, partial published, arrived,
+ * totals known; [5..8] pass B; [9..12] pass C; [13] decision).  Arm with
+ * xf_trace_next(ctx, buf, steps), run, then xf_trace_fetch(ctx, buf, steps).
+ * No counterpart in the reference. */
+int  xf_trace_next(xf_ctx* ctx, long long* out, int steps);
+int  xf_trace_fetch(xf_ctx* ctx, long long* out, int steps);
 /* Project autotuner: times candidate (block, cpt, resident) shapes for
  * `steps` steps each on a scratch copy of the state and keeps the fastest. */
 int  xf_autotune(xf_ctx* ctx, int steps, xf_opts* best, double* best_ms_per_step);

@@ -97,6 +97,8 @@ SIGNATURES = {
     "xf_set_launch": (_I, [_P, _I, _I, _I, _I]),
     "xf_launch_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "xf_autotune": (_I, [_P, _I, _OP, _DP]),
+    "xf_trace_next": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _I]),
+    "xf_trace_fetch": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _I]),
     "xf_ensemble_create": (_I, [_PP, _I, _OP, ctypes.POINTER(_P)]),
     "xf_ensemble_destroy": (_I, [_P]),
     "xf_ensemble_upload": (_I, [_P, _I, _DP, _D, _D]),
@@ -235,6 +237,15 @@ class Context:
         v = ctypes.c_int(); s = ctypes.c_int(); m = ctypes.c_int(); lds = ctypes.c_int()
         check(self.L.xf_launch_info(self.h, ctypes.byref(v), ctypes.byref(s), ctypes.byref(m), ctypes.byref(lds)))
         return dict(vgprs=v.value, scratch=s.value, max_blocks=m.value, lds=lds.value)
+
+    def trace(self, steps: int):
+        """Runs `steps` steps with per-phase timestamps; returns [steps, 16] in microseconds."""
+        buf = np.zeros((steps, 16), dtype=np.int64)
+        ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+        check(self.L.xf_trace_next(self.h, ptr, steps))
+        info, _ = self.run(steps)
+        check(self.L.xf_trace_fetch(self.h, ptr, steps))
+        return buf.astype(np.float64) * 0.01, info
 
     def autotune(self, steps: int = 20):
         o = XfOpts(); ms = ctypes.c_double()

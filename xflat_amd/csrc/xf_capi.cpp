@@ -98,6 +98,7 @@ struct XfInst {
         if (!(src_scale > 0.)) src_scale = 1.;
         std::vector<double> srcs(srcw.size());
         for (size_t i = 0; i < srcw.size(); ++i) srcs[i] = srcw[i] / src_scale;
+
         if ((rc = dput(&d_x, t.cos2R)) || (rc = dput(&d_dx, t.dcos2R)) || (rc = dput(&d_sinR, t.sinR)) ||
             (rc = dput(&d_cphi, t.cosphi)) || (rc = dput(&d_sphi, t.sinphi)) || (rc = dput(&d_hv, hv)) ||
             (rc = dput(&d_srcw, srcw)) || (rc = dput(&d_errw, errw)) || (rc = dput(&d_hvf, to_f(hv))) ||
@@ -121,10 +122,13 @@ struct XfInst {
         if (nblk <= part_nblk) return XF_OK;
         if (d_part) (void)hipFree(d_part);
         d_part = nullptr;
-        int rc = dalloc(&d_part, (size_t)3 * nblk * XF_NVMAX);
+        if (nblk > XF_BAR_GROUP * XF_BAR_MAXGROUPS)
+            return xf_fail(XF_EINVAL, "%d workgroups per instance exceed %d", nblk, XF_BAR_GROUP * XF_BAR_MAXGROUPS);
+        int rc = dalloc(&d_part, (size_t)(nblk + 1) * XF_NVMAX);
         if (rc) return rc;
         part_nblk = nblk;
         dev.part = d_part;
+        dev.tot = d_part + (size_t)nblk * XF_NVMAX;
         return XF_OK;
     }
 
@@ -141,6 +145,8 @@ struct XfInst {
         d.psi[0] = d_psi[0]; d.psi[1] = d_psi[1];
         d.rs = d_rs; d.bar = d_bar; d.part = d_part;
         d.log = nullptr;
+        d.trace = nullptr;
+        d.trace_steps = 0;
         d.dump_rstep = 0; d.dump_last_r = 0; d.dump_itrstep = 0;
     }
 
@@ -661,6 +667,42 @@ int xf_launch_info(const xf_ctx* ctx, int* vgprs, int* spill, int* max_blocks, i
         if (rc) return rc;
     }
     if (lds_bytes) *lds_bytes = (int)xf_step_smem_bytes(E->L, E->geom);
+    return XF_OK;
+}
+
+// Per-phase real-time stamps (100 MHz counter) of workgroup 0 for the first
+// `steps` steps of the next xf_run_steps; 16 slots per step: [0] step start,
+// [1..4] pass A cells done / partial published / arrived / totals known,
+// [5..8] the same for pass B, [9..12] pass C, [13] decision.
+int xf_trace_next(xf_ctx* ctx, long long* out, int steps) {
+    if (!ctx || steps < 0) return xf_fail(XF_EINVAL, "bad arguments");
+    XfInst* in = ctx->ens->inst[0].get();
+    static thread_local long long* d_tr = nullptr;
+    static thread_local int cap = 0;
+    if (!out || steps == 0) {
+        in->dev.trace = nullptr;
+        in->dev.trace_steps = 0;
+        return XF_OK;
+    }
+    if (steps > cap) {
+        if (d_tr) (void)hipFree(d_tr);
+        d_tr = nullptr;
+        int rc = dalloc(&d_tr, (size_t)steps * 16);
+        if (rc) return rc;
+        cap = steps;
+    }
+    HIPCHK(hipMemset(d_tr, 0, (size_t)steps * 16 * sizeof(long long)));
+    in->dev.trace = d_tr;
+    in->dev.trace_steps = steps;
+    return XF_OK;
+}
+int xf_trace_fetch(xf_ctx* ctx, long long* out, int steps) {
+    if (!ctx || !out) return xf_fail(XF_EINVAL, "bad arguments");
+    XfInst* in = ctx->ens->inst[0].get();
+    if (!in->dev.trace) return xf_fail(XF_EINVAL, "no trace armed");
+    HIPCHK(hipMemcpy(out, in->dev.trace, (size_t)std::min(steps, in->dev.trace_steps) * 16 * sizeof(long long), hipMemcpyDeviceToHost));
+    in->dev.trace = nullptr;
+    in->dev.trace_steps = 0;
     return XF_OK;
 }
 

@@ -8,6 +8,7 @@
 
 #include "../../include/xflat.h"
 
+
 // Constants of include/Util.h:15-22 and include/global.h:43-46.  The
 // reference truncates pi and sqrt(2); FP64 parity needs the same values.
 namespace xfc {
@@ -54,10 +55,21 @@ struct XfRunScalars {
     int pad;
 };
 
-struct XfGridBar {                 // generation barrier, one per instance
-    unsigned int count;
+// Cross-workgroup reduction of one instance: each workgroup writes its
+// partial sums to its own row of part[nblk][XF_NVMAX], then arrives on a
+// two-level tree of counters (groups of 16, then a top counter).  The
+// workgroup that arrives last overall gathers all rows, reduces them in a
+// fixed order, writes the totals and bumps the generation word; the others
+// poll that word with relaxed loads and read the totals.  Every counter sits
+// on its own 256-byte line.
+constexpr int XF_BAR_GROUP = 16;
+constexpr int XF_BAR_MAXGROUPS = 128;          // up to 2048 workgroups per instance
+struct XfGridBar {
+    unsigned int top;
+    unsigned int pad0[63];
     unsigned int gen;
-    unsigned int pad[62];          // own 256-byte line
+    unsigned int pad1[63];
+    unsigned int grp[XF_BAR_MAXGROUPS][64];
 };
 
 struct XfInstDev {
@@ -87,8 +99,12 @@ struct XfInstDev {
     void* psi[2];                  // [ns*4][ncells] planes, double or float
     XfRunScalars* rs;
     XfGridBar* bar;
-    double* part;                  // [3][nblk][NVMAX] partial sums (phases A,B,C)
+    double* part;                  // [nblk][XF_NVMAX] partial rows
+    double* tot;                   // [XF_NVMAX] reduced totals
     xf_step_rec* log;              // optional per-attempt log
+    long long* trace;              // optional [steps][16] real-time stamps of workgroup 0
+    int trace_steps;
+    int pad2;
     // stop predicates
     double dump_rstep, dump_last_r;
     int dump_itrstep;

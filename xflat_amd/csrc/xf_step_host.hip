@@ -27,7 +27,7 @@ size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
     int bsize = 3 + 3 * nm + 9 + 12;
     int nwaves = L.block / 64;
     return sizeof(double) * ((size_t)L.bpt * bsize + 4 * (size_t)L.block * L.cpt + 4 * (size_t)L.bpt +
-                             (size_t)nwaves * XF_NVMAX + XF_NVMAX);
+                             (size_t)nwaves * 8 * XF_NVMAX + XF_NVMAX);
 }
 
 int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {

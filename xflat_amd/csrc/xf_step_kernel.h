@@ -149,37 +149,53 @@ __device__ __forceinline__ void st_agent(double* p, double v) {
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Generation barrier over the nblk workgroups of one instance.  The wait is
-// bounded (2 s of the 100 MHz real-time counter); on timeout, or when another
-// workgroup has flagged an error, the status word is set and false returned,
-// so every workgroup leaves its step loop and the launch ends.
-__device__ __forceinline__ bool grid_sync(XfGridBar* bar, unsigned nblk, unsigned& gen, int* status, bool fence_all, int* s_flag) {
-    if (fence_all) __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        unsigned old = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == nblk - 1) {
-            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-            while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(1);
-                if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
-                if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
-                    __hip_atomic_store(status, (int)XF_EBARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    ok = 0;
-                    break;
-                }
+// Arrival on the tree barrier (thread 0 only).  Returns true for the
+// workgroup that arrived last overall; it must later call bar_release.
+// Fence-fence synchronisation: one agent release fence before arriving,
+// relaxed counter RMWs (which extend the release sequences) and an agent
+// acquire fence only in the workgroups that go on.  Group counters are reset
+// by their last arriver before it moves up, so they are clean before the
+// generation can change.
+__device__ __forceinline__ bool bar_arrive(XfGridBar* bar, int lb, int nblk) {
+    const int g = lb / XF_BAR_GROUP;
+    const int gsize = min(XF_BAR_GROUP, nblk - g * XF_BAR_GROUP);
+    const int ngroups = (nblk + XF_BAR_GROUP - 1) / XF_BAR_GROUP;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    unsigned old = __hip_atomic_fetch_add(&bar->grp[g][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old != (unsigned)(gsize - 1)) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __hip_atomic_store(&bar->grp[g][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    unsigned t = __hip_atomic_fetch_add(&bar->top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t != (unsigned)(ngroups - 1)) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __hip_atomic_store(&bar->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+__device__ __forceinline__ void bar_release(XfGridBar* bar, unsigned gen) {
+    __hip_atomic_store(&bar->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Bounded wait (thread 0 only) for the generation to move past `gen`: 2 s of
+// the 100 MHz real-time counter, or an error flagged by another workgroup.
+// Relaxed polling, so waiting workgroups do not invalidate caches while
+// others still compute; one acquire fence at the end.
+__device__ __forceinline__ bool bar_wait(XfGridBar* bar, unsigned gen, int* status) {
+    uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    bool ok = true;
+    int spins = 0;
+    while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if ((++spins & 15) == 0) {
+            if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = false; break; }
+            if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
+                __hip_atomic_store(status, (int)XF_EBARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = false;
+                break;
             }
         }
-        gen += 1u;
-        *s_flag = ok;
     }
-    __syncthreads();
-    if (fence_all) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return *s_flag != 0;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return ok;
 }
 
 // matt::getHm (Matt.cpp:31-64)
@@ -351,46 +367,99 @@ __device__ __forceinline__ Sp<double> candidate(const Sp<double>& p0, const Sp<f
     return nd;
 }
 
-// Block reduction of NV sums and NMAX maxima -> s_tot; thread 0 publishes
-// them as this workgroup's partial of phase `ph` and joins the barrier;
-// afterwards every workgroup sums all partials in the same fixed order.
+// Workgroup reduction of NV sums and NMAX maxima held one set per lane ->
+// s_tot.  Fixed structure (a butterfly over lane bits 5..3, then 16 lanes per
+// value over the per-wave partials in LDS), so equal inputs give equal bits.
 template <int NV, int NMAX>
-__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red, double* s_tot, const XfInstDev& I, int lb, int ph,
-                               unsigned& gen, int* status, bool fence_all, int* s_flag) {
+__device__ __forceinline__ void block_reduce(const double* acc, double* s_red, double* s_tot) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-    const int nblk = I.nblk;
+    constexpr int NT = NV + NMAX;
 #pragma unroll
-    for (int v = 0; v < NV + NMAX; ++v) {
-        double x = (v < NV) ? wave_sum(acc[v]) : wave_max(acc[v]);
-        if (lane == 0) s_red[wave * XF_NVMAX + v] = x;
-    }
-    __syncthreads();
-    if (tid < NV + NMAX) {
-        double x = s_red[tid];
-        for (int w = 1; w < nwaves; ++w)
-            x = (tid < NV) ? x + s_red[w * XF_NVMAX + tid] : fmax(x, s_red[w * XF_NVMAX + tid]);
-        s_tot[tid] = x;
-    }
-    __syncthreads();
-    double* part = I.part + ((long long)ph * nblk + lb) * XF_NVMAX;
-    if (tid == 0)
-        for (int v = 0; v < NV + NMAX; ++v) st_agent(part + v, s_tot[v]);
-    bool ok = grid_sync(I.bar, (unsigned)nblk, gen, status, fence_all, s_flag);
-    const double* pall = I.part + (long long)ph * nblk * XF_NVMAX;
-    for (int v = wave; v < NV + NMAX; v += nwaves) {
-        const bool is_max = v >= NV;
-        double x = 0.;
-        for (int j = lane; j < nblk; j += 64) {
-            double y = ld_agent(pall + (long long)j * XF_NVMAX + v);
-            x = is_max ? fmax(x, y) : x + y;
+    for (int v = 0; v < NT; ++v) {
+        double x = acc[v];
+#pragma unroll
+        for (int o = 32; o >= 8; o >>= 1) {
+            double y = __shfl_xor(x, o, 64);
+            x = (v < NV) ? x + y : fmax(x, y);
         }
-        x = is_max ? wave_max(x) : wave_sum(x);
-        if (lane == 0) s_tot[v] = x;
+        if (lane < 8) s_red[(v * nwaves + wave) * 8 + lane] = x;
     }
     __syncthreads();
-    return ok;
+    for (int t0 = 0; t0 < NT * 16; t0 += blockDim.x) {
+        const int t = t0 + tid;
+        const int v = t >> 4, j = t & 15;
+        double x = 0.;
+        if (v < NT) {
+            const double* src = s_red + v * nwaves * 8;
+            for (int i = j; i < nwaves * 8; i += 16) x = (v < NV) ? x + src[i] : fmax(x, src[i]);
+        }
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) {
+            double y = __shfl_xor(x, o, 16);
+            x = (v < NV) ? x + y : fmax(x, y);
+        }
+        if (v < NT && j == 0) s_tot[v] = x;
+    }
+    __syncthreads();
 }
 
+// Workgroup partial of NV sums and NMAX maxima -> row lb; tree arrival;
+// the last workgroup gathers all rows (thread j holds rows j, j + blockDim,
+// ... in registers, all loads in flight together), reduces them with the
+// fixed-structure block reduction, publishes the totals and releases the
+// others, who read the totals.  s_tot ends with the same bits everywhere.
+template <int NV, int NMAX>
+__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red, double* s_tot, const XfInstDev& I, int lb,
+                                               unsigned& gen, int* status, bool fence_all, int* s_flag,
+                                               long long* tr = nullptr) {
+    const int tid = threadIdx.x;
+    const int nblk = I.nblk;
+    constexpr int NT = NV + NMAX;
+    if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
+    block_reduce<NV, NMAX>(acc, s_red, s_tot);
+    if (fence_all) {
+        __threadfence();
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double* row = I.part + (long long)lb * XF_NVMAX;
+        for (int v = 0; v < NT; ++v) st_agent(row + v, s_tot[v]);
+        if (tr) tr[1] = (long long)__builtin_amdgcn_s_memrealtime();
+        s_flag[1] = bar_arrive(I.bar, lb, nblk) ? 1 : 0;
+    }
+    __syncthreads();
+    bool ok = true;
+    if (s_flag[1]) {
+        if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
+        double g[XF_NVMAX];
+#pragma unroll
+        for (int v = 0; v < NT; ++v) g[v] = 0.;
+        for (int j = tid; j < nblk; j += blockDim.x) {
+            const double* row = I.part + (long long)j * XF_NVMAX;
+#pragma unroll
+            for (int v = 0; v < NT; ++v) g[v] = (v < NV) ? g[v] + row[v] : fmax(g[v], row[v]);
+        }
+        block_reduce<NV, NMAX>(g, s_red, s_tot);
+        if (tid == 0) {
+            for (int v = 0; v < NT; ++v) st_agent(I.tot + v, s_tot[v]);
+            bar_release(I.bar, gen);
+        }
+    } else {
+        if (tid == 0) {
+            s_flag[0] = bar_wait(I.bar, gen, status) ? 1 : 0;
+            if (tr) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
+        }
+        __syncthreads();
+        ok = s_flag[0] != 0;
+        // plain loads: ordered after thread 0's acquire fence by the barrier
+        if (tid < NT) s_tot[tid] = I.tot[tid];
+    }
+    gen += 1u;
+    if (fence_all) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    if (tr && tid == 0) tr[3] = (long long)__builtin_amdgcn_s_memrealtime();
+    return ok;
+}
 }  // namespace
 
 // ============================================================================
@@ -424,9 +493,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     double* s_beam = smem;                                   // [bpt][R::SIZE]
     double* s_errb = s_beam + bpt * R::SIZE;                 // [4][B*CPT]
     double* s_errs = s_errb + 4 * B * CPT;                   // [bpt][4]
-    double* s_red = s_errs + 4 * bpt;                        // [nwaves][XF_NVMAX]
-    double* s_tot = s_red + nwaves * XF_NVMAX;               // [XF_NVMAX]
-    __shared__ int s_flag;
+    double* s_red = s_errs + 4 * bpt;                        // [XF_NVMAX][nwaves][8]
+    double* s_tot = s_red + nwaves * 8 * XF_NVMAX;           // [XF_NVMAX]
+    __shared__ int s_flag[2];
     // angular moments, uniform over the workgroup: psi0, psi1, psi3, psi4, candidate
     __shared__ double s_M0[12], s_M1[12], s_M3[12], s_M4[12], s_Mn[12];
 
@@ -435,7 +504,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     long long total = rs->total, accepted = rs->accepted;
     int cur = rs->cur;
     double last_err = rs->last_err;
-    unsigned gen = __hip_atomic_load(&I.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned gen = 0u;                    // generation of the instance barrier
     int* status = &rs->status;
 
     Cells<ST, CT, NS, CPT> C;
@@ -520,7 +589,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 acc_moments<NM>(acc, rec + R::W + 0 * NM, (double)X0, (double)X2, (double)X3);
             }
         }
-        ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, 2, gen, status, false, &s_flag);
+        ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, gen, status, false, s_flag);
         if (tid < NMV) M0[tid] = s_tot[tid];
         __syncthreads();
     }
@@ -534,6 +603,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         const double dr2 = .5 * dr;
         const double rdr = r + dr;
         const double rdr2 = r + dr2;
+        long long* tr = (I.trace && lb == 0 && steps_call < I.trace_steps) ? I.trace + steps_call * 16 : nullptr;
+        if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
         const double hm_r = hm_at(I, r), hm_rdr2 = hm_at(I, rdr2), hm_rdr = hm_at(I, rdr);
 
         // =================================================== pass A (S1, S2, S6)
@@ -584,7 +655,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
                 }
             }
-            ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, I, lb, 0, gen, status, false, &s_flag);
+            ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, I, lb, gen, status, false, s_flag, tr ? tr + 1 : nullptr);
             if (tid < NMV) { M1[tid] = s_tot[tid]; M3[tid] = s_tot[NMV + tid]; }
             __syncthreads();
             if (!ok) break;
@@ -642,7 +713,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
                 }
             }
-            ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, 1, gen, status, false, &s_flag);
+            ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, gen, status, false, s_flag, tr ? tr + 5 : nullptr);
             if (tid < NMV) M4[tid] = s_tot[tid];
             __syncthreads();
             if (!ok) break;
@@ -780,7 +851,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 }
             }
             acc[NMV] = blk_err;
-            ok = reduce_publish<NMV, 1>(acc, s_red, s_tot, I, lb, 2, gen, status, MODE == 2, &s_flag);
+            ok = reduce_publish<NMV, 1>(acc, s_red, s_tot, I, lb, gen, status, MODE == 2, s_flag, tr ? tr + 9 : nullptr);
             if (tid < NMV) Mn[tid] = s_tot[tid];
             errmax = s_tot[NMV];
             __syncthreads();
@@ -795,6 +866,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             rec.r = r; rec.dr = dr; rec.err = err; rec.accepted = acc_ok ? 1 : 0; rec.pad = 0;
             I.log[steps_call] = rec;
         }
+        if (tr && tid == 0) tr[13] = (long long)__builtin_amdgcn_s_memrealtime();
         const long long it = total;
         if (acc_ok) {
             r += dr;
