@@ -377,7 +377,8 @@ static int choose_shape(xf_ensemble* E) {
             int vg = 0, spill = 0;
             int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
             if (kmax <= 0) continue;
-            if (spill > 0 && mode != 2) continue;
+            // default shape without tuning: resident only when (almost) spill-free
+            if (spill > 64 && mode != 2) continue;
             int bpt = max_bpt(ne_max, cpt);
             while (bpt > 1 && block_for(bpt, ne_max, cpt) > kmax) --bpt;
             L.block = o.block > 0 ? std::min(o.block, kmax) : block_for(bpt, ne_max, cpt);
@@ -739,7 +740,6 @@ int xf_autotune(xf_ctx* ctx, int steps, xf_opts* best, double* best_ms_per_step)
                 int vg = 0, spill = 0;
                 int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
                 if (kmax <= 0 || L.block > kmax) continue;
-                if (spill > 0 && mode != 2) continue;
                 if (plan_shape(E, L, true) != XF_OK) continue;
                 // restore state, warm up, time
                 HIPCHK(hipMemcpy(in->d_psi[0], save, nst, hipMemcpyDeviceToDevice));

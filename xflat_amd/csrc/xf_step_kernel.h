@@ -53,14 +53,47 @@ template <class T> struct Sp { T ar, ai, br, bi; };
 // ---------------------------------------------------------------- FP64 path
 struct PropD { double m0, m1, m2, m3; };
 
+// lambda^2 = |H|^2 of the traceless 2x2 Hamiltonian (NBeam.cpp:366)
 __device__ __forceinline__ double lam_d(double h0, double h1, double h2) {
-    return sqrt(h0 * h0 + (h1 * h1 + h2 * h2));
+    return h0 * h0 + (h1 * h1 + h2 * h2);
 }
-__device__ __forceinline__ void prop_d(double h0, double h1, double h2, double lam, double dl, PropD& P) {
-    double ldr = lam * dl;
-    double s, c;
-    sincos(ldr, &s, &c);
-    double sc = s / lam;
+// U = cos(x) I - i sin(x)/lambda H with x = lambda dl (NBeam.cpp:360-393).
+// For x^2 < 1/16 both coefficients come from their Taylor series in x^2 =
+// lambda^2 dl^2 (cos x = 1 + sum_k (-1)^k x^2k/(2k)!, sin(x)/lambda = dl sum_k
+// (-1)^k x^2k/(2k+1)!, truncated after x^16: remainder < 1e-23): FMAs only, no
+// square root, division or transcendental call, which on CDNA4 (FP64 FMA at
+// the FP32 rate) is the cheap path.  Larger angles use sqrt/sincos/divide.
+__device__ __forceinline__ void prop_d(double h0, double h1, double h2, double lam2, double dl, PropD& P) {
+    const double u = lam2 * (dl * dl);
+    double c, sc;
+    if (u < 0.0625) {
+        double sn = 1.0 / 355687428096000.0;               // 1/17!
+        sn = fma(u, -sn, 1.0 / 1307674368000.0);           // 1/15!
+        sn = fma(u, sn, -1.0 / 6227020800.0);              // 1/13!
+        sn = fma(u, sn, 1.0 / 39916800.0);                 // 1/11!
+        sn = fma(u, sn, -1.0 / 362880.0);                  // 1/9!
+        sn = fma(u, sn, 1.0 / 5040.0);                     // 1/7!  (sign folded below)
+        sn = fma(u, -sn, 1.0 / 120.0);
+        sn = fma(u, -sn, 1.0 / 6.0) ;
+        // sn now holds 1/3! - u/5! + u^2/7! - ...; sinc = 1 - u sn
+        double cs = 1.0 / 20922789888000.0;                // 1/16!
+        cs = fma(u, -cs, 1.0 / 87178291200.0);             // 1/14!
+        cs = fma(u, cs, -1.0 / 479001600.0);               // 1/12!
+        cs = fma(u, cs, 1.0 / 3628800.0);                  // 1/10!
+        cs = fma(u, cs, -1.0 / 40320.0);                   // 1/8!
+        cs = fma(u, cs, 1.0 / 720.0);                      // 1/6!  (sign folded below)
+        cs = fma(u, -cs, 1.0 / 24.0);
+        cs = fma(u, -cs, 0.5);
+        // cs = 1/2! - u/4! + u^2/6! - ...; cos x = 1 - u cs
+        c = fma(-u, cs, 1.0);
+        sc = dl * fma(-u, sn, 1.0);
+    } else {
+        double lam = sqrt(lam2);
+        double ldr = lam * dl;
+        double s;
+        sincos(ldr, &s, &c);
+        sc = s / lam;
+    }
     P.m0 = c;
     P.m1 = -h0 * sc;
     P.m2 = h2 * sc;
