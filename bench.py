@@ -130,14 +130,26 @@ def cpu_baseline(psi0, budget_steps=None):
 
 
 def calibrate_copy(nbytes=1 << 30):
-    """A plain device copy of known size, for scaling rocprof FETCH/WRITE_SIZE."""
-    import torch
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
+    """A plain device-to-device copy of known size (the HIP runtime's own copy
+    kernel), for scaling rocprof FETCH_SIZE / WRITE_SIZE.  Calls the HIP runtime
+    directly so that a counter-collection run loads no other GPU library."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
     return nbytes
 
 

@@ -57,14 +57,13 @@ struct XfInst {
     float *d_hvf = nullptr, *d_srcwf = nullptr, *d_errwf = nullptr;
     void* d_psi[2] = {nullptr, nullptr};
     XfRunScalars* d_rs = nullptr;
-    XfGridBar* d_bar = nullptr;
-    double* d_part = nullptr;
-    int part_nblk = 0;
+    unsigned long long* d_rows = nullptr;
+    int rows_nblk = 0;
     XfInstDev dev{};
 
     ~XfInst() {
         void* ps[] = {d_x, d_dx, d_sinR, d_cphi, d_sphi, d_hv, d_srcw, d_errw, d_hvf, d_srcwf, d_errwf,
-                      d_psi[0], d_psi[1], d_rs, d_bar, d_part};
+                      d_psi[0], d_psi[1], d_rs, d_rows};
         for (void* q : ps)
             if (q) (void)hipFree(q);
     }
@@ -111,24 +110,22 @@ struct XfInst {
             if (e != hipSuccess) return xf_fail(XF_ENOMEM, "state buffer (%zu B): %s", nst * elem, hipGetErrorString(e));
             d_psi[b] = q;
         }
-        if ((rc = dalloc(&d_rs, 1)) || (rc = dalloc(&d_bar, 1))) return rc;
+        if ((rc = dalloc(&d_rs, 1))) return rc;
         HIPCHK(hipMemset(d_rs, 0, sizeof(XfRunScalars)));
-        HIPCHK(hipMemset(d_bar, 0, sizeof(XfGridBar)));
         fill_dev();
         return XF_OK;
     }
 
-    int ensure_partials(int nblk) {
-        if (nblk <= part_nblk) return XF_OK;
-        if (d_part) (void)hipFree(d_part);
-        d_part = nullptr;
-        if (nblk > XF_BAR_GROUP * XF_BAR_MAXGROUPS)
-            return xf_fail(XF_EINVAL, "%d workgroups per instance exceed %d", nblk, XF_BAR_GROUP * XF_BAR_MAXGROUPS);
-        int rc = dalloc(&d_part, (size_t)(nblk + 1) * XF_NVMAX);
+    // exchange rows for nblk workgroups: [2][XF_ROWW][nblk] words
+    int ensure_rows(int nblk) {
+        if (nblk <= rows_nblk) return XF_OK;
+        if (d_rows) (void)hipFree(d_rows);
+        d_rows = nullptr;
+        rows_nblk = 0;
+        int rc = dalloc(&d_rows, (size_t)2 * XF_ROWW * nblk);
         if (rc) return rc;
-        part_nblk = nblk;
-        dev.part = d_part;
-        dev.tot = d_part + (size_t)nblk * XF_NVMAX;
+        rows_nblk = nblk;
+        dev.rows = d_rows;
         return XF_OK;
     }
 
@@ -143,7 +140,7 @@ struct XfInst {
         d.nb0 = p.nb0; d.hNS = p.hNS; d.has_matter = p.has_matter; d.renorm = renorm;
         d.eps0 = p.eps0; d.ksi = p.ksi; d.max_dr = p.max_dr; d.Rn = p.Rn;
         d.psi[0] = d_psi[0]; d.psi[1] = d_psi[1];
-        d.rs = d_rs; d.bar = d_bar; d.part = d_part;
+        d.rs = d_rs; d.rows = d_rows;
         d.log = nullptr;
         d.trace = nullptr;
         d.trace_steps = 0;
@@ -305,24 +302,37 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     }
     XfLaunch Lq = L;
     Lq.bpt = max_bpt_used;              // LDS per workgroup depends on the largest tile
-    int maxb = 0;
-    int rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
-    if (rc) return rc;
-    for (size_t i = 0; i < E->inst.size(); ++i) {
-        XfInst* in = E->inst[i].get();
-        int nt = tiles[i];
-        if (L.mode != 2) {
-            nblk[i] = nt;
-        } else {
-            // share the co-resident workgroups in proportion to the cells
-            long long share = (long long)maxb * in->ncells() / std::max<long long>(1, total_cells);
-            if (L.grid > 0) share = std::min<long long>(share, (long long)L.grid * in->ncells() / std::max<long long>(1, total_cells));
-            nblk[i] = (int)std::max<long long>(1, std::min<long long>(nt, share));
+    Lq.xrows = 1;                       // ... and on the exchange staging (rows of one instance)
+    int maxb = 0, rc = 0;
+    // the staging grows with the workgroup count and may lower the occupancy:
+    // settle both (the counts only shrink)
+    for (int pass = 0; pass < 4; ++pass) {
+        rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
+        if (rc) return rc;
+        grid = 0;
+        int xrows = 1;
+        for (size_t i = 0; i < E->inst.size(); ++i) {
+            XfInst* in = E->inst[i].get();
+            int nt = tiles[i];
+            if (L.mode != 2) {
+                nblk[i] = nt;
+            } else {
+                // share the co-resident workgroups in proportion to the cells
+                long long share = (long long)maxb * in->ncells() / std::max<long long>(1, total_cells);
+                if (L.grid > 0) share = std::min<long long>(share, (long long)L.grid * in->ncells() / std::max<long long>(1, total_cells));
+                nblk[i] = (int)std::max<long long>(1, std::min<long long>(nt, share));
+            }
+            grid += nblk[i];
+            xrows = std::max(xrows, nblk[i]);
         }
-        grid += nblk[i];
+        if (xrows <= Lq.xrows) break;
+        Lq.xrows = xrows;
     }
+    rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
+    if (rc) return rc;
     if (grid > maxb)
         return xf_fail(XF_EINVAL, "shape needs %d co-resident workgroups, device holds %d", grid, maxb);
+    L.xrows = Lq.xrows;
     if (!commit) {
         L.grid = grid;
         return XF_OK;
@@ -332,7 +342,7 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     int b0 = 0;
     for (size_t i = 0; i < E->inst.size(); ++i) {
         XfInst* in = E->inst[i].get();
-        rc = in->ensure_partials(nblk[i]);
+        rc = in->ensure_rows(nblk[i]);
         if (rc) return rc;
         in->dev.bpt = bpts[i];
         in->dev.ntiles = tiles[i];
@@ -411,10 +421,10 @@ static int ens_launch(xf_ensemble* E, int max_steps, xf_step_rec* log, double* m
         d.log = nullptr;
         HIPCHK(hipMemcpyAsync(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice, E->stream));
     }
-    for (auto& in : E->inst) {
-        HIPCHK(hipMemsetAsync(in->d_bar, 0, sizeof(XfGridBar), E->stream));
-        HIPCHK(hipMemsetAsync(&in->d_rs->status, 0, sizeof(int), E->stream));
-    }
+    if (3ll * max_steps + 1 > XF_MAX_EXCHANGES)
+        return xf_fail(XF_EINVAL, "%d steps in one launch exceed the exchange tags", max_steps);
+    int rc0 = xf_step_prepare(E->d_insts, (int)E->inst.size(), E->stream);
+    if (rc0) return rc0;
     HIPCHK(hipEventRecord(E->ev0, E->stream));
     int rc = xf_step_launch_geom(E->L, E->geom, E->d_insts, E->d_blk_inst, max_steps, E->stream);
     if (rc) return rc;

@@ -55,22 +55,17 @@ struct XfRunScalars {
     int pad;
 };
 
-// Cross-workgroup reduction of one instance: each workgroup writes its
-// partial sums to its own row of part[nblk][XF_NVMAX], then arrives on a
-// two-level tree of counters (groups of 16, then a top counter).  The
-// workgroup that arrives last overall gathers all rows, reduces them in a
-// fixed order, writes the totals and bumps the generation word; the others
-// poll that word with relaxed loads and read the totals.  Every counter sits
-// on its own 256-byte line.
-constexpr int XF_BAR_GROUP = 16;
-constexpr int XF_BAR_MAXGROUPS = 128;          // up to 2048 workgroups per instance
-struct XfGridBar {
-    unsigned int top;
-    unsigned int pad0[63];
-    unsigned int gen;
-    unsigned int pad1[63];
-    unsigned int grp[XF_BAR_MAXGROUPS][64];
-};
+// Cross-workgroup reduction of one instance (all-gather of tagged rows):
+// each workgroup publishes its partial sums as one row of 64-bit words, value
+// v split into its low and high 32-bit halves (words 2v, 2v + 1), each word
+// carrying the exchange's generation + 1 in its upper 32 bits.  Rows are
+// value-major, rows[parity][word][nblk], so word w of all rows is one
+// contiguous array.  Every workgroup fetches every row and reduces them in
+// the same order.  The buffer is cleared before each launch (generations
+// restart at 0); a launch performs at most XF_MAX_EXCHANGES exchanges.
+constexpr int XF_NVMAX = 32;                   // max values per partial row
+constexpr int XF_ROWW = 2 * XF_NVMAX;          // words per row
+constexpr long long XF_MAX_EXCHANGES = 0xF0000000ll;
 
 struct XfInstDev {
     int module, nth, nph, ne, nbeam;
@@ -98,10 +93,8 @@ struct XfInstDev {
     // state and work buffers
     void* psi[2];                  // [ns*4][ncells] planes, double or float
     XfRunScalars* rs;
-    XfGridBar* bar;
-    double* part;                  // [nblk][XF_NVMAX] partial rows
-    double* tot;                   // [XF_NVMAX] reduced totals
-    xf_step_rec* log;              // optional per-attempt log
+    unsigned long long* rows;      // [2][XF_ROWW][nblk] tagged partial rows
+    xf_step_rec* log;             // optional per-attempt log
     long long* trace;              // optional [steps][16] real-time stamps of workgroup 0
     int trace_steps;
     int pad2;
@@ -110,8 +103,6 @@ struct XfInstDev {
     int dump_itrstep;
     int pad;
 };
-
-constexpr int XF_NVMAX = 32;       // max values per block partial
 
 // ---- launch configuration ---------------------------------------------------
 struct XfLaunch {
@@ -122,6 +113,7 @@ struct XfLaunch {
     int block;         // threads per workgroup
     int bpt;           // beams per tile
     int grid;          // workgroups (all instances)
+    int xrows;         // largest workgroup count of one instance (exchange staging in LDS)
 };
 
 // Launch helpers of the fused stepper (xf_step_host.hip).
@@ -131,3 +123,6 @@ int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks);
 int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int max_steps,
                         void* stream);
 int xf_step_func_info(const XfLaunch& L, int geom, int* vgprs, int* spill_bytes);
+// Clears the run status and the exchange rows of every instance of a launch
+// (a kernel, so the launch sequence holds no memset node).
+int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream);

@@ -1,6 +1,7 @@
 // xf_step_host.hip -- launch helpers of the fused stepper: kernel choice,
 // LDS size, co-residency, cooperative launch.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "xf_internal.h"
 
 using XfStepFn = void (*)(const XfInstDev*, const int*, int);
@@ -26,8 +27,9 @@ size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
     int nm = geom == XF_MAA ? 4 : 2;
     int bsize = 3 + 3 * nm + 9 + 12;
     int nwaves = L.block / 64;
-    return sizeof(double) * ((size_t)L.bpt * bsize + 4 * (size_t)L.block * L.cpt + 4 * (size_t)L.bpt +
-                             (size_t)nwaves * 8 * XF_NVMAX + XF_NVMAX);
+    // the pass-C error buffer doubles as the exchange staging [2 * 3 * nm][xrows]
+    size_t errb = std::max((size_t)4 * L.block * L.cpt, (size_t)6 * nm * std::max(L.xrows, 1));
+    return sizeof(double) * ((size_t)L.bpt * bsize + errb + 4 * (size_t)L.bpt + (size_t)nwaves * 8 * XF_NVMAX + XF_NVMAX);
 }
 
 int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {
@@ -74,5 +76,23 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
     hipError_t e = hipLaunchCooperativeKernel((const void*)f, dim3(L.grid), dim3(L.block), args, smem, (hipStream_t)stream);
     if (e != hipSuccess)
         return xf_fail(XF_EHIP, "cooperative launch (grid %d x %d, %zu B LDS): %s", L.grid, L.block, smem, hipGetErrorString(e));
+    return XF_OK;
+}
+
+// Status and exchange rows of every instance to zero (generation tags start
+// at 1, so a cleared word matches no exchange).
+__global__ void k_step_prepare(const XfInstDev* __restrict__ insts) {
+    const XfInstDev& I = insts[blockIdx.y];
+    if (blockIdx.x == 0 && threadIdx.x == 0) I.rs->status = 0;
+    const long long n = 2ll * XF_ROWW * I.nblk;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        I.rows[i] = 0ull;
+}
+
+int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream) {
+    if (ninst <= 0 || ninst > 65535) return xf_fail(XF_EINVAL, "%d instances in one launch", ninst);
+    hipLaunchKernelGGL(k_step_prepare, dim3(8, ninst), dim3(256), 0, (hipStream_t)stream, d_insts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return xf_fail(XF_EHIP, "prepare launch: %s", hipGetErrorString(e));
     return XF_OK;
 }

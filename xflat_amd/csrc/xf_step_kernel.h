@@ -18,9 +18,10 @@
 // (S1..S11 of Nmr.cpp:177-206).  Between passes each workgroup reduces its
 // angular-moment partials (2 moments x 3 density components for MA,
 // Phy_MA.cpp:388-446; 4 x 3 for MAA, Phy_MAA.cpp:468-562) in a fixed order,
-// a grid barrier publishes them, and every workgroup sums all partials in the
-// same fixed order -- so all workgroups hold bit-identical moments, error,
-// accept/reject (Nmr.cpp:227) and dr (Nmr.cpp:247-249) without another sync.
+// publishes them as a tagged row, and every workgroup gathers all rows and
+// sums them in the same fixed order -- so all workgroups hold bit-identical
+// moments, error, accept/reject (Nmr.cpp:227) and dr (Nmr.cpp:247-249)
+// without another sync.
 //
 // State: structure-of-arrays planes psi[species*4 + comp][cell], cell =
 // beam*NE + e, double or float.  With 2 species the nu_x / anti-nu_x columns
@@ -47,6 +48,9 @@ namespace {
 
 constexpr double INV_2PI_D = 0.15915494309189535;
 constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;   // 2 s at 100 MHz
+#ifndef XF_EXCHANGE_WORDS
+#define XF_EXCHANGE_WORDS 8      // words per thread in flight in the exchange fetch
+#endif
 
 template <class T> struct Sp { T ar, ai, br, bi; };
 
@@ -173,62 +177,25 @@ __device__ __forceinline__ double wave_max(double v) {
     for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
-__device__ __forceinline__ double ld_agent(const double* p) {
-    unsigned long long u = __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)u);
-}
-__device__ __forceinline__ void st_agent(double* p, double v) {
-    __hip_atomic_store((unsigned long long*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
 
-// Arrival on the tree barrier (thread 0 only).  Returns true for the
-// workgroup that arrived last overall; it must later call bar_release.
-// Fence-fence synchronisation: one agent release fence before arriving,
-// relaxed counter RMWs (which extend the release sequences) and an agent
-// acquire fence only in the workgroups that go on.  Group counters are reset
-// by their last arriver before it moves up, so they are clean before the
-// generation can change.
-__device__ __forceinline__ bool bar_arrive(XfGridBar* bar, int lb, int nblk) {
-    const int g = lb / XF_BAR_GROUP;
-    const int gsize = min(XF_BAR_GROUP, nblk - g * XF_BAR_GROUP);
-    const int ngroups = (nblk + XF_BAR_GROUP - 1) / XF_BAR_GROUP;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    unsigned old = __hip_atomic_fetch_add(&bar->grp[g][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old != (unsigned)(gsize - 1)) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __hip_atomic_store(&bar->grp[g][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    unsigned t = __hip_atomic_fetch_add(&bar->top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (t != (unsigned)(ngroups - 1)) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __hip_atomic_store(&bar->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+// Tagged row words of the cross-workgroup exchange: 32-bit generation tag
+// (exchange index + 1) over one 32-bit half of a partial sum.
+__device__ __forceinline__ unsigned long long ld_word(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void bar_release(XfGridBar* bar, unsigned gen) {
-    __hip_atomic_store(&bar->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_word(unsigned long long* p, unsigned long long w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Bounded wait (thread 0 only) for the generation to move past `gen`: 2 s of
-// the 100 MHz real-time counter, or an error flagged by another workgroup.
-// Relaxed polling, so waiting workgroups do not invalidate caches while
-// others still compute; one acquire fence at the end.
-__device__ __forceinline__ bool bar_wait(XfGridBar* bar, unsigned gen, int* status) {
-    uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-    bool ok = true;
-    int spins = 0;
-    while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(1);
-        if ((++spins & 15) == 0) {
-            if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = false; break; }
-            if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
-                __hip_atomic_store(status, (int)XF_EBARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = false;
-                break;
-            }
-        }
+// Bounded-wait bookkeeping: 2 s of the 100 MHz real-time counter, or an
+// error flagged by another workgroup.  Returns true when the wait must end.
+__device__ __forceinline__ bool wait_expired(uint64_t t0, int spins, int* status) {
+    if ((spins & 15) != 0) return false;
+    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+    if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
+        __hip_atomic_store(status, (int)XF_EBARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return ok;
+    return false;
 }
 
 // matt::getHm (Matt.cpp:31-64)
@@ -436,62 +403,107 @@ __device__ __forceinline__ void block_reduce(const double* acc, double* s_red, d
     __syncthreads();
 }
 
-// Workgroup partial of NV sums and NMAX maxima -> row lb; tree arrival;
-// the last workgroup gathers all rows (thread j holds rows j, j + blockDim,
-// ... in registers, all loads in flight together), reduces them with the
-// fixed-structure block reduction, publishes the totals and releases the
-// others, who read the totals.  s_tot ends with the same bits everywhere.
+// Cross-workgroup reduction of NV sums and NMAX maxima (maxima of values
+// >= 0) by an all-gather of tagged rows:
+//  1. the workgroup reduces its lanes (fixed structure) and publishes the
+//     result as row lb: value v -> words 2v (low half) and 2v + 1 (high
+//     half), each tagged with gen + 1, in rows[gen & 1][word][nblk];
+//  2. wave 0 polls word 0 of all rows (one contiguous array) until every
+//     row carries the tag;
+//  3. the workgroup fetches all words (coalesced), checks every tag, stages
+//     the payloads in LDS and sums each value over the rows in a fixed
+//     order, which leaves the same bits in every workgroup.
+// The rows are the only data that crosses workgroups, so no fence is needed:
+// each word is one single-copy-atomic 64-bit store and its tag names its
+// exchange.  Two parities suffice because a workgroup publishes exchange
+// gen + 2 only after every row of gen + 1 arrived, and a workgroup publishes
+// gen + 1 only after its fetch of gen returned (the published values depend
+// on the fetched ones).  All waits are bounded.
 template <int NV, int NMAX>
-__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red, double* s_tot, const XfInstDev& I, int lb,
-                                               unsigned& gen, int* status, bool fence_all, int* s_flag,
-                                               long long* tr = nullptr) {
-    const int tid = threadIdx.x;
+__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red, double* s_tot, double* s_x, const XfInstDev& I,
+                                               int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr) {
+    const int tid = threadIdx.x, lane = tid & 63;
     const int nblk = I.nblk;
     constexpr int NT = NV + NMAX;
+    const unsigned tag = gen + 1u;
+    const unsigned long long tagw = (unsigned long long)tag << 32;
+    unsigned long long* rows = I.rows + (size_t)(gen & 1u) * XF_ROWW * nblk;
     if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
     block_reduce<NV, NMAX>(acc, s_red, s_tot);
-    if (fence_all) {
-        __threadfence();
-        __syncthreads();
+    if (tid < 2 * NT) {
+        const unsigned long long u = (unsigned long long)__double_as_longlong(s_tot[tid >> 1]);
+        const unsigned half = (tid & 1) ? (unsigned)(u >> 32) : (unsigned)u;
+        st_word(rows + (size_t)tid * nblk + lb, tagw | half);
     }
-    if (tid == 0) {
-        double* row = I.part + (long long)lb * XF_NVMAX;
-        for (int v = 0; v < NT; ++v) st_agent(row + v, s_tot[v]);
-        if (tr) tr[1] = (long long)__builtin_amdgcn_s_memrealtime();
-        s_flag[1] = bar_arrive(I.bar, lb, nblk) ? 1 : 0;
+    if (tr && tid == 0) tr[1] = (long long)__builtin_amdgcn_s_memrealtime();
+    if (tid < 64) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        int fail = 0;
+        for (int spins = 1;; ++spins) {
+            int mine = 1;
+            for (int j = lane; j < nblk; j += 64) mine &= (unsigned)(ld_word(rows + j) >> 32) == tag;
+            if (__all(mine)) break;
+            if (wait_expired(t0, spins, status)) { fail = 1; break; }
+        }
+        fail = __any(fail);
+        if (tid == 0) s_flag[0] = fail;
     }
     __syncthreads();
-    bool ok = true;
-    if (s_flag[1]) {
-        if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
-        double g[XF_NVMAX];
+    int fail = s_flag[0];
+    // fetch: word p of the value-major array (p = q * nblk + j: half q of
+    // row j), coalesced, XW words per thread in flight; its payload goes to
+    // s_h[p].  A stale tag (a word that had not landed when word 0 had) makes
+    // the whole workgroup fetch again.
+    constexpr int XW = XF_EXCHANGE_WORDS;
+    const int nw = 2 * NT * nblk;
+    const int B = blockDim.x;
+    unsigned* s_h = reinterpret_cast<unsigned*>(s_x);
+    const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
+    for (int spins = 1; !fail; ++spins) {
+        int bad = 0;
+        for (int p0 = tid; p0 < nw; p0 += XW * B) {
+            unsigned long long w[XW];
 #pragma unroll
-        for (int v = 0; v < NT; ++v) g[v] = 0.;
-        for (int j = tid; j < nblk; j += blockDim.x) {
-            const double* row = I.part + (long long)j * XF_NVMAX;
+            for (int k = 0; k < XW; ++k) {
+                const int p = p0 + k * B;
+                w[k] = (p < nw) ? ld_word(rows + p) : tagw;
+            }
 #pragma unroll
-            for (int v = 0; v < NT; ++v) g[v] = (v < NV) ? g[v] + row[v] : fmax(g[v], row[v]);
+            for (int k = 0; k < XW; ++k) {
+                const int p = p0 + k * B;
+                bad |= (unsigned)(w[k] >> 32) != tag;
+                if (p < nw) s_h[p] = (unsigned)w[k];
+            }
         }
-        block_reduce<NV, NMAX>(g, s_red, s_tot);
-        if (tid == 0) {
-            for (int v = 0; v < NT; ++v) st_agent(I.tot + v, s_tot[v]);
-            bar_release(I.bar, gen);
-        }
-    } else {
-        if (tid == 0) {
-            s_flag[0] = bar_wait(I.bar, gen, status) ? 1 : 0;
-            if (tr) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
-        }
-        __syncthreads();
-        ok = s_flag[0] != 0;
-        // plain loads: ordered after thread 0's acquire fence by the barrier
-        if (tid < NT) s_tot[tid] = I.tot[tid];
+        if (!__syncthreads_or(bad)) break;
+        fail = __syncthreads_or(wait_expired(t1, 16 * spins, status));
     }
+    if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
+    // per value: 16 lanes sum rows j = l, l + 16, ... in order, then a
+    // butterfly over the 16 lanes -- the same bits in every workgroup
+    for (int t0 = 0; t0 < NT * 16; t0 += B) {
+        const int t = t0 + tid;
+        const int v = t >> 4, l = t & 15;
+        double x = 0.;
+        if (v < NT && !fail) {
+            const unsigned* lo = s_h + (size_t)(2 * v) * nblk;
+            const unsigned* hi = lo + nblk;
+            for (int j = l; j < nblk; j += 16) {
+                const double y = __longlong_as_double((long long)(((unsigned long long)hi[j] << 32) | lo[j]));
+                x = (v < NV) ? x + y : fmax(x, y);
+            }
+        }
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) {
+            const double y = __shfl_xor(x, o, 16);
+            x = (v < NV) ? x + y : fmax(x, y);
+        }
+        if (v < NT && l == 0) s_tot[v] = x;
+    }
+    __syncthreads();
     gen += 1u;
-    if (fence_all) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    __syncthreads();
     if (tr && tid == 0) tr[3] = (long long)__builtin_amdgcn_s_memrealtime();
-    return ok;
+    return !fail;
 }
 }  // namespace
 
@@ -524,8 +536,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 
     extern __shared__ double smem[];
     double* s_beam = smem;                                   // [bpt][R::SIZE]
-    double* s_errb = s_beam + bpt * R::SIZE;                 // [4][B*CPT]
-    double* s_errs = s_errb + 4 * B * CPT;                   // [bpt][4]
+    double* s_errb = s_beam + bpt * R::SIZE;                 // [4][B*CPT], between passes the
+                                                             // exchange's staging [2*NMV][nblk]
+    double* s_errs = s_errb + max(4 * B * CPT, 2 * NMV * nblk);   // [bpt][4]
     double* s_red = s_errs + 4 * bpt;                        // [XF_NVMAX][nwaves][8]
     double* s_tot = s_red + nwaves * 8 * XF_NVMAX;           // [XF_NVMAX]
     __shared__ int s_flag[2];
@@ -622,7 +635,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 acc_moments<NM>(acc, rec + R::W + 0 * NM, (double)X0, (double)X2, (double)X3);
             }
         }
-        ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, gen, status, false, s_flag);
+        ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag);
         if (tid < NMV) M0[tid] = s_tot[tid];
         __syncthreads();
     }
@@ -688,7 +701,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
                 }
             }
-            ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, I, lb, gen, status, false, s_flag, tr ? tr + 1 : nullptr);
+            ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
             if (tid < NMV) { M1[tid] = s_tot[tid]; M3[tid] = s_tot[NMV + tid]; }
             __syncthreads();
             if (!ok) break;
@@ -746,7 +759,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
                 }
             }
-            ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, I, lb, gen, status, false, s_flag, tr ? tr + 5 : nullptr);
+            ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
             if (tid < NMV) M4[tid] = s_tot[tid];
             __syncthreads();
             if (!ok) break;
@@ -884,7 +897,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 }
             }
             acc[NMV] = blk_err;
-            ok = reduce_publish<NMV, 1>(acc, s_red, s_tot, I, lb, gen, status, MODE == 2, s_flag, tr ? tr + 9 : nullptr);
+            ok = reduce_publish<NMV, 1>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
             if (tid < NMV) Mn[tid] = s_tot[tid];
             errmax = s_tot[NMV];
             __syncthreads();
