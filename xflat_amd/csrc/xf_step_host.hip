@@ -25,7 +25,7 @@ bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nu
 
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
     int nm = geom == XF_MAA ? 4 : 2;
-    int bsize = 3 + 3 * nm + 9 + 12;
+    int bsize = 3 + 3 * nm + 9 + 12 + 4;
     int nwaves = L.block / 64;
     // the pass-C error buffer doubles as the exchange staging [2 * 3 * nm][xrows]
     size_t errb = std::max((size_t)4 * L.block * L.cpt, (size_t)6 * nm * std::max(L.xrows, 1));

@@ -131,6 +131,38 @@ __device__ __forceinline__ void prop_f(float h0, float h1, float h2, float dl, P
     P.m2 = h2 * sc;
     P.m3 = -h1 * sc;
 }
+// prop_f plus the coefficients of the remainder R = (U - I) + i dl H, i.e.
+// (cos x - 1) I - i (sin(x)/lambda - dl) H: second order in x = lambda dl, its
+// sin(x)/lambda - dl = dl (sin(x)/x - 1) from the series -x^2/6 + x^4/120 -
+// x^6/5040 for x < 1/4 (next term < 5e-9 relative), so its FP32 rounding is
+// relative to x^2, not to x.
+__device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, PropF& P, PropF& PR) {
+    float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
+    float rl = lam2 > 0.f ? __builtin_amdgcn_rsqf(lam2) : 0.f;
+    float lam = lam2 * rl;
+    float xr = lam * dl;
+    float x = xr * (float)(0.5 * INV_2PI_D);                // half angle, revolutions
+    x = x - __builtin_rintf(x);
+    float sh = __builtin_amdgcn_sinf(x);
+    float ch = __builtin_amdgcn_cosf(x);
+    float s = 2.f * sh * ch;
+    float sc = s * rl;
+    float scr;
+    if (xr < 0.25f) {
+        float x2 = xr * xr;
+        scr = dl * x2 * fmaf(x2, fmaf(x2, -1.f / 5040.f, 1.f / 120.f), -1.f / 6.f);
+    } else {
+        scr = sc - dl;
+    }
+    P.cm1 = -2.f * sh * sh;
+    P.m1 = -h0 * sc;
+    P.m2 = h2 * sc;
+    P.m3 = -h1 * sc;
+    PR.cm1 = P.cm1;
+    PR.m1 = -h0 * scr;
+    PR.m2 = h2 * scr;
+    PR.m3 = -h1 * scr;
+}
 __device__ __forceinline__ Sp<float> delta_f(const PropF& P, const Sp<float>& a) {
     Sp<float> o;
     o.ar = (P.cm1 * a.ar - P.m1 * a.ai) + (P.m2 * a.br - P.m3 * a.bi);
@@ -214,7 +246,8 @@ template <int GEOM> struct BRec {
     static constexpr int W = 3;                   // W[3 pts][NM] moment weights
     static constexpr int G = W + 3 * NM;          // G[3 pts][3] hvv geometry
     static constexpr int H = G + 9;               // Hb[4][3]: h0, h1, h3, h4 (+matter)
-    static constexpr int SIZE = H + 12;
+    static constexpr int HD = H + 12;             // dl0 Hb0 + dl2 Hb4 + dl1 (Hb3 - Hb0 - Hb1), dl0 + dl2 - dl1
+    static constexpr int SIZE = HD + 4;
 };
 
 // Geometry of one beam at r, r+dr/2, r+dr: angle bins (Phy_MA.cpp:172-196,
@@ -289,6 +322,23 @@ __device__ __forceinline__ void beam_h(const double* rec, int pt, const double* 
     Hb[2] = hv[2];
 }
 
+// First-order combination of the step's four Hamiltonians that enters
+// psi4 - psi1 (see the compensated error of the mixed stepper): per beam the
+// part of sum_k c_k dl_k H_k that does not depend on the energy, and the
+// coefficient sum_k c_k dl_k of the vacuum part.  FP64, after h4 is known.
+template <int GEOM>
+__device__ __forceinline__ void beam_hd(double* rec) {
+    using R = BRec<GEOM>;
+    const double dl0 = rec[R::DL + 0], dl1 = rec[R::DL + 1], dl2 = rec[R::DL + 2];
+    const double* B0 = rec + R::H + 0;
+    const double* B1 = rec + R::H + 3;
+    const double* B3 = rec + R::H + 6;
+    const double* B4 = rec + R::H + 9;
+#pragma unroll
+    for (int z = 0; z < 3; ++z) rec[R::HD + z] = dl0 * B0[z] + dl2 * B4[z] + dl1 * ((B3[z] - B0[z]) - B1[z]);
+    rec[R::HD + 3] = (dl0 + dl2) - dl1;
+}
+
 // Per-cell data of one thread.
 template <class ST, class CT, int NS, int CPT> struct Cells {
     static constexpr bool F64 = std::is_same<CT, double>::value;
@@ -356,12 +406,18 @@ __device__ __forceinline__ void acc_moments(double* acc, const double* W, double
 
 
 // psi0 + increment, formed in FP64; with renorm the unitarity of the state
-// |a|^2 + |b|^2 = 1 is restored (FP32 modes only).
+// |a|^2 + |b|^2 = 1 is restored (FP32 modes only).  The norm deviates from 1
+// by the FP32 rounding of one increment (~1e-7), so 1/sqrt(1 + d) comes from
+// its series 1 - d/2 + 3d^2/8 - 5d^3/16 (remainder < 35 d^4/128 < 3e-17 for
+// |d| < 1e-4); a larger deviation takes sqrt and divide.
 __device__ __forceinline__ Sp<double> candidate(const Sp<double>& p0, const Sp<float>& df, int renorm) {
     Sp<double> nd{p0.ar + (double)df.ar, p0.ai + (double)df.ai, p0.br + (double)df.br, p0.bi + (double)df.bi};
     if (renorm) {
-        double n2 = (nd.ar * nd.ar + nd.ai * nd.ai) + (nd.br * nd.br + nd.bi * nd.bi);
-        double sc = 1.0 / sqrt(n2);
+        const double n2 = (nd.ar * nd.ar + nd.ai * nd.ai) + (nd.br * nd.br + nd.bi * nd.bi);
+        const double d = n2 - 1.0;
+        double sc;
+        if (fabs(d) < 1e-4) sc = fma(d, fma(d, fma(d, -0.3125, 0.375), -0.5), 1.0);
+        else sc = 1.0 / sqrt(n2);
         nd.ar *= sc; nd.ai *= sc; nd.br *= sc; nd.bi *= sc;
     }
     return nd;
@@ -514,6 +570,9 @@ template <class ST, class CT, int NS, int GEOM, int CPT, int MODE, int MAXB>
 __global__ void __launch_bounds__(MAXB)
 k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, int max_steps) {
     constexpr bool F64 = std::is_same<CT, double>::value;
+    // mixed precision (FP64 state, FP32 increments): compensated error estimate
+    constexpr bool COMP = !F64 && std::is_same<ST, double>::value;
+    static_assert(!(COMP && MODE == 0), "the mixed stepper recomputes its increments in pass C");
     using R = BRec<GEOM>;
     constexpr int NM = R::NM;
     constexpr int NMV = NM * 3;
@@ -785,6 +844,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 }
                 __syncthreads();
                 tile_h(nbt, 3, 2, M4, hm_rdr);
+                if constexpr (COMP)
+                    for (int b = tid; b < nbt; b += B) beam_hd<GEOM>(s_beam + b * R::SIZE);
                 __syncthreads();
 #pragma unroll
                 for (int k = 0; k < CPT; ++k) {
@@ -833,7 +894,47 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             if (MODE == 2) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), fin);
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
-                            Sp<float> d1, d3, d4;
+                            Sp<float> d1, d3, d4, df;
+                            if constexpr (COMP) {
+                                // increments of the state as below; the error from the
+                                // remainders (second order, so rounded relative to x^2)
+                                // plus the first-order part -i Hd psi0 with Hd combined
+                                // per beam in FP64:  psi4 - psi1 = E / 2 with
+                                // E = R0 + R4(psi0 + d3) + L4 d3 + R3 - R1 - Rb - i Hd psi0
+                                double x0, x1, x2;
+                                PropF P, PR;
+                                cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);                    // h0
+                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
+                                Sp<float> E = delta_f(PR, a);
+                                E = Sp<float>{-E.ar, -E.ai, -E.br, -E.bi};
+                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 0], P, PR);
+                                d3 = delta_f(P, a);
+                                E = sp_add(E, delta_f(PR, a));
+                                cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);                    // h1
+                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
+                                Sp<float> t = delta_f(PR, a);
+                                E = Sp<float>{E.ar - t.ar, E.ai - t.ai, E.br - t.br, E.bi - t.bi};
+                                cell_h(I, e, s, rec + R::H + 6, x0, x1, x2);                    // h3
+                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
+                                d4 = delta_f(P, a);
+                                E = sp_add(E, delta_f(PR, a));
+                                const float dl2 = (float)rec[R::DL + 2];
+                                const float g0 = (float)h0, g1 = (float)h1, g2 = (float)h2;    // h4
+                                prop_f2(g0, g1, g2, dl2, P, PR);
+                                const Sp<float> b3 = sp_add(a, d3);
+                                const Sp<float> d3n = sp_add(d3, delta_f(P, b3));
+                                E = sp_add(E, delta_f(PR, b3));
+                                const PropF PL{0.f, -g0 * dl2, g2 * dl2, -g1 * dl2};
+                                E = sp_add(E, delta_f(PL, d3));
+                                const double* hd = rec + R::HD;
+                                const double hv0 = I.hv[e], hv1 = I.hv[I.ne + e];
+                                const double sg = (s & 1) ? -1.0 : 1.0;
+                                const PropF PD{0.f, (float)-fma(hd[3], hv0, sg * hd[0]), (float)hd[2],
+                                               (float)-fma(hd[3], hv1, sg * hd[1])};
+                                E = sp_add(E, delta_f(PD, a));
+                                df = sp_avg(d3n, d4);
+                                dsq = .25 * (double)((E.ar * E.ar + E.ai * E.ai) + (E.br * E.br + E.bi * E.bi));
+                            } else {
                             if (MODE == 0) {
                                 d1 = C.k1[k][s]; d3 = C.k3[k][s]; d4 = C.k4[k][s];
                             } else {
@@ -854,8 +955,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             PropF P2;
                             prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 2], P2);
                             Sp<float> d3n = sp_add(d3, delta_f(P2, sp_add(a, d3)));
-                            Sp<float> df = sp_avg(d3n, d4);
+                            df = sp_avg(d3n, d4);
                             dsq = (double)dnorm2<float>(df, d1);
+                            }
                             Sp<double> nd = candidate(sp_d(C.p0[k][s]), df, I.renorm);
                             Sp<float> vf = sp_f(nd);
                             src_add<CT>(I, e, s, vf, X0, X2, X3);
