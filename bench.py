@@ -129,6 +129,25 @@ def cpu_baseline(psi0, budget_steps=None):
                        f"(OpenMP {ncores} threads, {secs:.2f} s, host '{model}')")
 
 
+PROFILE_HBM = os.path.join(ROOT, "profiles", "k_step_mixed_ma_1000x400_hbm.md")
+PROFILED_LAUNCH = (1, 4, 5)          # mode, cells per lane, beams per tile of that profile
+
+
+def profiled_traffic_per_step(precision, launch):
+    """HBM bytes per attempted step measured by rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (scaled by a copy kernel) for the launch shape of the committed profile;
+    None for any other precision or shape."""
+    from xflat_amd import capi
+    if precision != capi.XF_MIXED or (launch["mode"], launch["cpt"], launch["beams_per_tile"]) != PROFILED_LAUNCH:
+        return None
+    try:
+        with open(PROFILE_HBM) as f:
+            m = re.search(r"^traffic_bytes_per_step:\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def calibrate_copy(nbytes=1 << 30):
     """A plain device-to-device copy of known size (the HIP runtime's own copy
     kernel), for scaling rocprof FETCH_SIZE / WRITE_SIZE.  Calls the HIP runtime
@@ -181,6 +200,8 @@ def main():
     bytes_per = 64 if prec != capi.XF_FP64 else 128
     kern_s = res["kernel_ms"] / 1e3
     achieved = bytes_per * cells * res["steps"] / kern_s / 1e9
+    tps = profiled_traffic_per_step(prec, res["launch"])
+    traffic = tps * res["steps"] if tps else None
     out = {
         "metric": "beam-steps/sec (angle x energy x dr) at stated P_ee error vs FP64; % HBM roofline",
         "value": value,
@@ -201,11 +222,13 @@ def main():
         "radius_km": res["r"],
         "kernel_ms_per_step": res["kernel_ms"] / max(1, res["steps"]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": (os.path.relpath(PROFILE_HBM, ROOT) if traffic else None),
                      "note": (f"algorithmic {bytes_per} B per beam-step (nu + anti-nu 2-spinors read and written once) "
                               "over the stepper kernel's HIP-event time; the state stays in registers across steps "
                               "(resident launch), so the step is bound by the latency of its three dependent "
-                              "grid-wide moment reductions, not by HBM")},
+                              "grid-wide moment reductions, not by HBM; traffic = HBM bytes of the timed launch "
+                              "from the committed FETCH_SIZE/WRITE_SIZE profile of the same launch shape")},
     }
     print(f"[bench] {a.precision}: {value:.3e} beam-steps/s, {out['ms_per_step']:.4f} ms/step, "
           f"launch {res['launch']}", file=sys.stderr, flush=True)
@@ -217,6 +240,7 @@ def main():
         pb = lambda s: s[:, 1, 0] ** 2 + s[:, 1, 1] ** 2
         out["fp64"] = {"value": fp64_value, "ms_per_step": 1e3 * ref["wall_s"] / max(1, ref["steps"]),
                        "accepted_steps": ref["accepted"], "radius_km": ref["r"], "launch": ref["launch"]}
+        out["radius_advance_vs_fp64"] = (res["r"] - R0) / max(ref["r"] - R0, 1e-300)
         out["pee_error_vs_fp64"] = {
             "note": "after the same number of attempted steps from the same state; radii differ when the "
                     "accepted counts differ",

@@ -205,6 +205,32 @@ def test_mixed_precision_close_to_fp64(prec):
     assert norm <= 1e-6
 
 
+def test_mixed_step_sequence_follows_fp64():
+    """At eps0 = 1e-9 near the neutrinosphere the FP32 rounding of a first-order
+    increment (~6e-8 x) is comparable to the tolerance; the mixed stepper's
+    compensated error estimate keeps its accept/reject and dr sequence on the
+    FP64 one, so it advances as far per attempted step."""
+    capi = _capi()
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import seeded_psi
+    p = bulb_params(Abins=200, Ebins=100, R0=10, Rn=1000, eps0="1.E-9", dm2="-2.4E-3")
+    psi0 = seeded_psi(200, 100, seed=7, amp=1e-6)
+    recs, radii = {}, {}
+    for name, prec in (("fp64", capi.XF_FP64), ("mixed", capi.XF_MIXED)):
+        with capi.Context(p, precision=prec) as c:
+            c.upload(psi0, 10.0, 1e-3)
+            recs[name] = _run_all(c, 400)
+            radii[name] = c.download()[1]
+    f, m = recs["fp64"], recs["mixed"]
+    agree = np.mean(f["accepted"] == m["accepted"])
+    ratio = np.median(m["dr"][50:] / f["dr"][50:])
+    adv = (radii["mixed"] - 10.0) / (radii["fp64"] - 10.0)
+    print(f"accept agreement {agree:.4f}, median dr ratio {ratio:.4f}, radius advance ratio {adv:.4f}")
+    assert agree >= 0.99
+    assert abs(ratio - 1.0) < 0.03
+    assert adv > 0.95
+
+
 def test_ensemble_instances_match_single_runs():
     capi = _capi()
     from oracle.oracle import Oracle
