@@ -108,6 +108,8 @@ typedef struct xf_run_info {
     int64_t total_accepted; /* counter since the state was set                 */
     double  r, dr;          /* radius and next step length after the call      */
     double  last_err;       /* error of the last attempted step                */
+    double  last_dr;        /* step length of the last accepted step (the dr
+                               the reference writes with a dump, Nmr.cpp:238)  */
     double  kernel_ms;      /* device time of the call (HIP events)            */
     int32_t stop_reason;    /* 0 budget, 1 r reached Rn, 2 dump due, 3 error   */
     int32_t pad;

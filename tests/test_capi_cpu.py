@@ -1,5 +1,5 @@
 """CPU checks of the C-ABI library: it loads without a GPU, exports every
-entry point include/xflat.h declares, and its config reader (Parser.cpp
+entry point include/*.h declares, and its config reader (Parser.cpp
 keywords and unit scaling) agrees with the Python one."""
 import os
 import re
@@ -13,8 +13,12 @@ from xflat_amd.config import config_text, parse_config_text, MEV, EV2, ERG_S
 
 
 def _declared():
-    txt = open(os.path.join(ROOT, "include", "xflat.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*|double)\s+(xf_[a-z_0-9]+)\(", txt, re.M)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"^\s*(?:int|const char\*|double)\s+(xf_[a-z_0-9]+)\(", txt, re.M))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():

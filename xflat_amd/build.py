@@ -1,4 +1,5 @@
-"""Builds the in-tree HIP library xflat_amd/libxflat.so for gfx950.
+"""Builds the in-tree HIP library xflat_amd/libxflat.so for gfx950 and the
+host programs over its C-ABI (xflat_amd/bin/).
 
 Every translation unit under xflat_amd/csrc is compiled by hipcc in
 parallel (object files under xflat_amd/_build/), then linked into one shared
@@ -16,6 +17,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libxflat.so")
+HOST = os.path.join(HERE, "host")
+BIN = os.path.join(HERE, "bin")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 ARCH = os.environ.get("XFLAT_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -52,7 +57,25 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+def build_host() -> list[str]:
+    """Host programs over the C-ABI (xflat_amd/host/*.cpp -> xflat_amd/bin/)."""
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    cxx = shutil.which("g++") or "c++"
+    for src in sorted(f for f in os.listdir(HOST) if f.endswith(".cpp")):
+        exe = os.path.join(BIN, src[:-4])
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, os.path.join(HOST, src), "-o", exe + ".tmp",
+               "-L", HERE, "-lxflat", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath-link,{ROCM_LIB}"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed on {src}:\n{r.stderr[-4000:]}")
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

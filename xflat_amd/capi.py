@@ -44,7 +44,7 @@ STEP_DTYPE = np.dtype([("r", "<f8"), ("dr", "<f8"), ("err", "<f8"), ("accepted",
 class XfRunInfo(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("accepted", ctypes.c_int64), ("total_steps", ctypes.c_int64),
                 ("total_accepted", ctypes.c_int64), ("r", ctypes.c_double), ("dr", ctypes.c_double),
-                ("last_err", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+                ("last_err", ctypes.c_double), ("last_dr", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("stop_reason", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
@@ -107,6 +107,16 @@ SIGNATURES = {
     "xf_ensemble_run_steps": (_I, [_P, _I, ctypes.POINTER(XfRunInfo)]),
     "xf_ensemble_cells": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "xf_ensemble_launch": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    # include/xflat_io.h (host-side NetCDF output and resume)
+    "xf_snapshot_create": (_I, [_PP, ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "xf_snapshot_append": (_I, [_P, ctypes.c_double, ctypes.c_double, _DP]),
+    "xf_probability_create": (_I, [_PP, ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "xf_probability_append": (_I, [_P, ctypes.c_double, _DP, _DP]),
+    "xf_nc_records": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "xf_nc_sync": (_I, [_P]),
+    "xf_nc_close": (_I, [_P]),
+    "xf_snapshot_read_last": (_I, [ctypes.c_char_p, _PP, _DP, _DP, _DP]),
+    "xf_output_name": (_I, [_PP, _I, _I, _I, ctypes.c_char_p, ctypes.c_size_t]),
 }
 
 
@@ -350,3 +360,58 @@ class Ensemble:
         m = ctypes.c_int(); c = ctypes.c_int(); b = ctypes.c_int(); g = ctypes.c_int()
         check(self.L.xf_ensemble_launch(self.h, ctypes.byref(m), ctypes.byref(c), ctypes.byref(b), ctypes.byref(g)))
         return dict(mode=m.value, cpt=c.value, block=b.value, grid=g.value)
+
+
+class NcFile:
+    """Snapshot or Probability output file (include/xflat_io.h)."""
+
+    def __init__(self, p: XfParams, path: str, probability: bool = False):
+        self.L = load()
+        self.p = p
+        self.h = ctypes.c_void_p()
+        fn = self.L.xf_probability_create if probability else self.L.xf_snapshot_create
+        check(fn(ctypes.byref(p), path.encode(), ctypes.byref(self.h)))
+
+    def append_snapshot(self, r: float, dr: float, psi: np.ndarray):
+        a = np.ascontiguousarray(psi, dtype=np.float64)
+        check(self.L.xf_snapshot_append(self.h, r, dr, dp(a)))
+
+    def append_probability(self, r: float, psi: np.ndarray, fv: np.ndarray):
+        a = np.ascontiguousarray(psi, dtype=np.float64)
+        w = np.ascontiguousarray(fv, dtype=np.float64)
+        check(self.L.xf_probability_append(self.h, r, dp(a), dp(w)))
+
+    def records(self) -> int:
+        n = ctypes.c_int64()
+        check(self.L.xf_nc_records(self.h, ctypes.byref(n)))
+        return n.value
+
+    def sync(self):
+        check(self.L.xf_nc_sync(self.h))
+
+    def close(self):
+        if self.h:
+            check(self.L.xf_nc_close(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def snapshot_read_last(path: str, p: XfParams):
+    """Last record (psi, r, dr) of a Snapshot file (the resume read)."""
+    nb = p.abins * (p.pbins if p.module == 1 else 1)
+    psi = np.zeros((nb, 4, 4, p.ebins))
+    r = ctypes.c_double()
+    dr = ctypes.c_double()
+    check(load().xf_snapshot_read_last(path.encode(), ctypes.byref(p), dp(psi), ctypes.byref(r), ctypes.byref(dr)))
+    return psi, r.value, dr.value
+
+
+def output_name(p: XfParams, probability: bool, ctr: int = 0, rank: int = 0) -> str:
+    buf = ctypes.create_string_buffer(1024)
+    check(load().xf_output_name(ctypes.byref(p), int(probability), ctr, rank, buf, 1024))
+    return buf.value.decode()

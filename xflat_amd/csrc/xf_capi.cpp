@@ -481,6 +481,7 @@ static int fill_info(XfInst* in, xf_run_info* info, double ms) {
         info->r = rs.r;
         info->dr = rs.dr;
         info->last_err = rs.last_err;
+        info->last_dr = rs.last_dr;
         info->kernel_ms = ms;
         info->stop_reason = rs.status ? 3 : rs.stop_reason;
     }

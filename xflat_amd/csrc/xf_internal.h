@@ -48,6 +48,7 @@ struct XfRunScalars {
     double r, dr;
     long long total, accepted;     // counter_tot, counter (Nmr.cpp:123)
     double last_err;
+    double last_dr;                // dr of the last accepted step
     int cur;                       // state buffer holding psi0 (streaming)
     int status;                    // 0 ok, XF_EBARRIER ...
     long long steps_call, acc_call;

@@ -609,6 +609,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     long long total = rs->total, accepted = rs->accepted;
     int cur = rs->cur;
     double last_err = rs->last_err;
+    double last_dr = rs->last_dr;
     unsigned gen = 0u;                    // generation of the instance barrier
     int* status = &rs->status;
 
@@ -1040,6 +1041,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         ++total;
         ++steps_call;
         last_err = err;
+        if (acc_ok) last_dr = dr;
         dr *= I.ksi * sqrt(I.eps0 / err);
         dr = (dr > I.max_dr) ? I.max_dr : dr;
         dr = (r + dr > I.Rn) ? I.Rn - r : dr;
@@ -1070,6 +1072,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         rs->accepted = accepted;
         rs->cur = cur;
         rs->last_err = last_err;
+        rs->last_dr = last_dr;
         rs->steps_call = steps_call;
         rs->acc_call = acc_call;
         rs->stop_reason = stop_reason;
