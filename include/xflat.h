@@ -228,6 +228,8 @@ int  xf_ensemble_upload(xf_ensemble* ens, int i, const double* psi, double r, do
 int  xf_ensemble_init_pure(xf_ensemble* ens, int i);
 int  xf_ensemble_download(xf_ensemble* ens, int i, double* psi, double* r, double* dr);
 int  xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info /*n*/);
+/* Launch-shape tuner over all instances (as xf_autotune; states restored). */
+int  xf_ensemble_autotune(xf_ensemble* ens, int steps, xf_opts* best, double* best_ms_per_step);
 int  xf_ensemble_cells(const xf_ensemble* ens, int64_t* cells);
 int  xf_ensemble_launch(const xf_ensemble* ens, int* mode, int* cpt, int* block, int* grid);
 

@@ -106,19 +106,45 @@ def test_fused_chunked_equals_single_launch():
         assert np.array_equal(a.download()[0], b.download()[0])
 
 
-def test_fused_modes_agree():
-    """Register-resident and streaming launches of the FP64 stepper."""
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "fp32"])
+def test_fused_modes_agree(prec):
+    """The resident-recompute and streaming launches of one stepper run the
+    same code: the same results to rounding of the final state.  With 4 cells
+    per lane the per-lane moment sums group the cells differently (FP64
+    rounding ~1e-16 that the neutrinosphere dynamics grow to ~1e-8 in the
+    phases over 200 steps), and the resident-keep launch forms the error
+    remainders in passes A/B (the compiler may contract them into FMAs
+    differently): FP64 takes the same accepted steps and keeps P_ee to the
+    parity bar; the FP32-increment steppers resolve err to ~u/x relative
+    (x = lambda dl), so a decision within that of eps0 can flip: >= 95 % of the
+    decisions agree and P_ee stays within 1e-5."""
     capi = _capi()
+    pr = {"fp64": capi.XF_FP64, "mixed": capi.XF_MIXED, "fp32": capi.XF_FP32}[prec]
     g, p = _golden("ma_r10_ih.npz")
     out = {}
-    for mode in (1, 2):
-        with capi.Context(p, precision=capi.XF_FP64) as ctx:
+    shapes = ((1, 1), (2, 1), (1, 4)) + (((0, 1),) if prec != "fp64" else ())
+    for mode, cpt in shapes:
+        with capi.Context(p, precision=pr) as ctx:
             ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
-            ctx.set_launch(mode, 1, 64 * ((8 * p.ebins + 63) // 64), 8)
+            ctx.set_launch(mode, cpt, 64 * ((8 * p.ebins // cpt + 63) // 64), 8)
             rec = _run_all(ctx, 200)
-            out[mode] = (rec, ctx.download()[0])
-    assert np.array_equal(np.nonzero(out[1][0]["accepted"])[0], np.nonzero(out[2][0]["accepted"])[0])
-    assert np.abs(out[1][1] - out[2][1]).max() <= 1e-12
+            out[(mode, cpt)] = (rec, ctx.download()[0])
+    a = out[(1, 1)]
+    for key in shapes[1:]:
+        b = out[key]
+        pa, pb = _pee(a[1]), _pee(b[1])
+        dpee = max(np.abs(pa[0] - pb[0]).max(), np.abs(pa[1] - pb[1]).max())
+        print(f"[{prec}] {key}: decisions agree {np.mean(a[0]['accepted'] == b[0]['accepted']):.3f}, "
+              f"max |dP_ee| {dpee:.2e}")
+        if key == (2, 1) or prec == "fp64":
+            assert np.array_equal(a[0]["accepted"], b[0]["accepted"]), key
+        if key == (2, 1):
+            assert np.abs(a[1] - b[1]).max() <= 1e-12, key
+        elif prec == "fp64":
+            assert dpee <= PEE_TOL and np.abs(a[1] - b[1]).max() <= 1e-7, key
+        else:
+            assert np.mean(a[0]["accepted"] == b[0]["accepted"]) >= 0.95, key
+            assert dpee <= 1e-5, key
 
 
 def test_phy_mirror_one_step_matches_oracle():
@@ -205,19 +231,21 @@ def test_mixed_precision_close_to_fp64(prec):
     assert norm <= 1e-6
 
 
-def test_mixed_step_sequence_follows_fp64():
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_mixed_step_sequence_follows_fp64(prec):
     """At eps0 = 1e-9 near the neutrinosphere the FP32 rounding of a first-order
-    increment (~6e-8 x) is comparable to the tolerance; the mixed stepper's
-    compensated error estimate keeps its accept/reject and dr sequence on the
-    FP64 one, so it advances as far per attempted step."""
+    increment (~6e-8 x) is comparable to the tolerance; the compensated error
+    estimate of the FP32-increment steppers keeps their accept/reject and dr
+    sequence on the FP64 one, so they advance as far per attempted step."""
     capi = _capi()
+    pr = capi.XF_MIXED if prec == "mixed" else capi.XF_FP32
     from xflat_amd.config import bulb_params
     from xflat_amd.seed import seeded_psi
     p = bulb_params(Abins=200, Ebins=100, R0=10, Rn=1000, eps0="1.E-9", dm2="-2.4E-3")
     psi0 = seeded_psi(200, 100, seed=7, amp=1e-6)
     recs, radii = {}, {}
-    for name, prec in (("fp64", capi.XF_FP64), ("mixed", capi.XF_MIXED)):
-        with capi.Context(p, precision=prec) as c:
+    for name, prc in (("fp64", capi.XF_FP64), ("mixed", pr)):
+        with capi.Context(p, precision=prc) as c:
             c.upload(psi0, 10.0, 1e-3)
             recs[name] = _run_all(c, 400)
             radii[name] = c.download()[1]

@@ -105,6 +105,7 @@ SIGNATURES = {
     "xf_ensemble_init_pure": (_I, [_P, _I]),
     "xf_ensemble_download": (_I, [_P, _I, _DP, _DP, _DP]),
     "xf_ensemble_run_steps": (_I, [_P, _I, ctypes.POINTER(XfRunInfo)]),
+    "xf_ensemble_autotune": (_I, [_P, _I, _OP, _DP]),
     "xf_ensemble_cells": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "xf_ensemble_launch": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     # include/xflat_io.h (host-side NetCDF output and resume)
@@ -355,6 +356,12 @@ class Ensemble:
         c = ctypes.c_int64()
         check(self.L.xf_ensemble_cells(self.h, ctypes.byref(c)))
         return c.value
+
+    def autotune(self, steps=20):
+        o = XfOpts()
+        ms = ctypes.c_double()
+        check(self.L.xf_ensemble_autotune(self.h, steps, ctypes.byref(o), ctypes.byref(ms)))
+        return o, ms.value
 
     def launch(self):
         m = ctypes.c_int(); c = ctypes.c_int(); b = ctypes.c_int(); g = ctypes.c_int()

@@ -374,8 +374,8 @@ static int choose_shape(xf_ensemble* E) {
     for (auto& in : E->inst) ne_max = std::max(ne_max, in->t.ne);
     std::vector<int> modes, cpts;
     if (o.resident == 0) modes = {2};
-    else if (o.resident == 1) modes = (o.precision == XF_FP64) ? std::vector<int>{1} : std::vector<int>{0, 1};
-    else modes = (o.precision == XF_FP64) ? std::vector<int>{1, 2} : std::vector<int>{0, 1, 2};
+    else if (o.resident == 1) modes = {0, 1};
+    else modes = {0, 1, 2};
     if (o.cpt > 0) cpts = {o.cpt};
     else cpts = {1, 2, 4};
     for (int mode : modes) {
@@ -486,6 +486,108 @@ static int fill_info(XfInst* in, xf_run_info* info, double ms) {
         info->stop_reason = rs.status ? 3 : rs.stop_reason;
     }
     if (rs.status) return xf_fail(rs.status, "stepper status %d (grid barrier did not complete)", rs.status);
+    return XF_OK;
+}
+
+// Times every instantiated (mode, cells per lane, beams per tile) on the
+// ensemble's current states for `steps` attempted steps and keeps the
+// fastest; every instance's state and run scalars are restored afterwards.
+static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_ms_per_step) {
+    struct Saved { void* psi = nullptr; size_t bytes = 0; XfRunScalars rs{}; };
+    std::vector<Saved> sv(E->inst.size());
+    int rc = XF_OK;
+    int ne_max = 1;
+    for (size_t i = 0; i < E->inst.size(); ++i) {
+        XfInst* in = E->inst[i].get();
+        ne_max = std::max(ne_max, in->t.ne);
+        sv[i].bytes = (size_t)in->ns * 4 * in->ncells() * in->elem;
+        if ((rc = in->scalars(&sv[i].rs))) break;
+        if (hipMalloc(&sv[i].psi, sv[i].bytes) != hipSuccess) { rc = xf_fail(XF_ENOMEM, "autotune: state copy"); break; }
+        if (hipMemcpy(sv[i].psi, in->d_psi[sv[i].rs.cur], sv[i].bytes, hipMemcpyDeviceToDevice) != hipSuccess) {
+            rc = xf_fail(XF_EHIP, "autotune: state copy");
+            break;
+        }
+    }
+    auto restore = [&]() -> int {
+        for (size_t i = 0; i < E->inst.size(); ++i) {
+            XfInst* in = E->inst[i].get();
+            if (!sv[i].psi) continue;
+            HIPCHK(hipMemcpy(in->d_psi[0], sv[i].psi, sv[i].bytes, hipMemcpyDeviceToDevice));
+            XfRunScalars rs = sv[i].rs;
+            rs.cur = 0;
+            HIPCHK(hipMemcpy(in->d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
+        }
+        return XF_OK;
+    };
+    double best_t = 1e300;
+    XfLaunch bestL{};
+    bool found = false;
+    const XfLaunch prev = E->L;
+    const bool had_shape = E->shaped;
+    for (int mode = 0; rc == XF_OK && mode <= 2; ++mode)
+        for (int cpt : {1, 2, 4}) {
+            XfLaunch L{};
+            L.precision = E->o.precision;
+            L.ns = E->inst[0]->ns;
+            L.mode = mode;
+            L.cpt = cpt;
+            if (!xf_step_has_shape(L, E->geom)) continue;
+            // beams per tile: the eight largest (they decide how evenly the
+            // beams spread over the co-resident workgroups), then halvings
+            int bmax = max_bpt(ne_max, cpt);
+            std::vector<int> bpts;
+            for (int b = bmax; b >= 1 && b > bmax - 8; --b) bpts.push_back(b);
+            for (int b = (bmax - 8) / 2; b >= 1; b /= 2) bpts.push_back(b);
+            for (int bpt : bpts) {
+                L.bpt = bpt;
+                L.block = block_for(bpt, ne_max, cpt);
+                L.grid = 0;
+                int vg = 0, spill = 0;
+                int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
+                if (kmax <= 0 || L.block > kmax) continue;
+                XfLaunch Lp = L;
+                if (plan_shape(E, Lp, true) != XF_OK) continue;
+                if ((rc = restore())) break;
+                double ms = 0;
+                if (ens_launch(E, 2, nullptr, &ms) != XF_OK) continue;
+                if (ens_launch(E, steps, nullptr, &ms) != XF_OK) continue;
+                bool good = true;
+                long long done = 0;
+                for (auto& in : E->inst) {
+                    XfRunScalars r2;
+                    if (in->scalars(&r2) != XF_OK || r2.status) { good = false; break; }
+                    done = std::max<long long>(done, r2.steps_call);
+                }
+                if (!good || done <= 0) continue;
+                const double per = ms / (double)done;
+                if (per < best_t) {
+                    best_t = per;
+                    bestL = E->L;
+                    found = true;
+                }
+            }
+        }
+    int rc2 = restore();
+    for (auto& x : sv)
+        if (x.psi) (void)hipFree(x.psi);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (!found) {
+        if (had_shape) { XfLaunch L = prev; plan_shape(E, L, true); }
+        return xf_fail(XF_EINVAL, "autotune: no shape ran");
+    }
+    XfLaunch L = bestL;
+    rc = plan_shape(E, L, true);
+    if (rc) return rc;
+    if (best) {
+        *best = E->o;
+        best->species = E->inst[0]->ns;
+        best->block = E->L.block;
+        best->cpt = E->L.cpt;
+        best->resident = E->L.mode;
+        best->grid = E->L.grid;
+    }
+    if (best_ms_per_step) *best_ms_per_step = best_t;
     return XF_OK;
 }
 
@@ -720,77 +822,7 @@ int xf_trace_fetch(xf_ctx* ctx, long long* out, int steps) {
 
 int xf_autotune(xf_ctx* ctx, int steps, xf_opts* best, double* best_ms_per_step) {
     if (!ctx || steps <= 0) return xf_fail(XF_EINVAL, "xf_autotune: bad arguments");
-    xf_ensemble* E = ctx->ens;
-    XfInst* in = E->inst[0].get();
-    // save the state
-    const size_t nst = (size_t)in->ns * 4 * in->ncells() * in->elem;
-    XfRunScalars rs0;
-    int rc = in->scalars(&rs0);
-    if (rc) return rc;
-    void* save = nullptr;
-    HIPCHK(hipMalloc(&save, nst));
-    HIPCHK(hipMemcpy(save, in->d_psi[rs0.cur], nst, hipMemcpyDeviceToDevice));
-    double best_t = 1e300;
-    XfLaunch bestL{};
-    bool found = false;
-    const int ne = in->t.ne;
-    for (int mode = 0; mode <= 2; ++mode)
-        for (int cpt : {1, 2, 4}) {
-            XfLaunch L{};
-            L.precision = E->o.precision;
-            L.ns = in->ns;
-            L.mode = mode;
-            L.cpt = cpt;
-            if (!xf_step_has_shape(L, E->geom)) continue;
-            int bmax = max_bpt(ne, cpt);
-            std::vector<int> bpts = {bmax};
-            for (int b = bmax / 2; b >= 1; b /= 2) bpts.push_back(b);
-            for (int bpt : bpts) {
-                L.bpt = bpt;
-                L.block = block_for(bpt, ne, cpt);
-                int vg = 0, spill = 0;
-                int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
-                if (kmax <= 0 || L.block > kmax) continue;
-                if (plan_shape(E, L, true) != XF_OK) continue;
-                // restore state, warm up, time
-                HIPCHK(hipMemcpy(in->d_psi[0], save, nst, hipMemcpyDeviceToDevice));
-                XfRunScalars rs = rs0;
-                rs.cur = 0;
-                HIPCHK(hipMemcpy(in->d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
-                double ms = 0;
-                if (ens_launch(E, 2, nullptr, &ms) != XF_OK) continue;
-                if (ens_launch(E, steps, nullptr, &ms) != XF_OK) continue;
-                XfRunScalars r2;
-                in->scalars(&r2);
-                if (r2.status || r2.steps_call <= 0) continue;
-                double per = ms / (double)r2.steps_call;
-                if (per < best_t) {
-                    best_t = per;
-                    bestL = E->L;
-                    found = true;
-                }
-            }
-        }
-    // restore state and the winning shape
-    HIPCHK(hipMemcpy(in->d_psi[0], save, nst, hipMemcpyDeviceToDevice));
-    XfRunScalars rs = rs0;
-    rs.cur = 0;
-    HIPCHK(hipMemcpy(in->d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
-    (void)hipFree(save);
-    if (!found) return xf_fail(XF_EINVAL, "autotune: no shape ran");
-    XfLaunch L = bestL;
-    rc = plan_shape(E, L, true);
-    if (rc) return rc;
-    if (best) {
-        *best = E->o;
-        best->species = in->ns;
-        best->block = E->L.block;
-        best->cpt = E->L.cpt;
-        best->resident = E->L.mode;
-        best->grid = E->L.grid;
-    }
-    if (best_ms_per_step) *best_ms_per_step = best_t;
-    return XF_OK;
+    return ens_autotune(ctx->ens, steps, best, best_ms_per_step);
 }
 
 // ---- ensemble
@@ -822,6 +854,11 @@ int xf_ensemble_download(xf_ensemble* ens, int i, double* psi, double* r, double
     if (!ens || i < 0 || i >= (int)ens->inst.size()) return xf_fail(XF_EINVAL, "bad instance");
     return ens->inst[i]->download(psi, r, dr);
 }
+int xf_ensemble_autotune(xf_ensemble* ens, int steps, xf_opts* best, double* best_ms_per_step) {
+    if (!ens || steps <= 0) return xf_fail(XF_EINVAL, "xf_ensemble_autotune: bad arguments");
+    return ens_autotune(ens, steps, best, best_ms_per_step);
+}
+
 int xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info) {
     if (!ens || max_steps < 0) return xf_fail(XF_EINVAL, "bad arguments");
     double ms = 0;

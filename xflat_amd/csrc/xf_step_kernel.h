@@ -171,6 +171,16 @@ __device__ __forceinline__ Sp<float> delta_f(const PropF& P, const Sp<float>& a)
     o.bi = (-P.m2 * a.ai + P.m3 * a.ar) + (P.cm1 * a.bi - P.m1 * a.br);
     return o;
 }
+// (U - I) is linear in its coefficients: sums of coefficient sets apply as one.
+__device__ __forceinline__ PropF pf_add(const PropF& p, const PropF& q) {
+    return PropF{p.cm1 + q.cm1, p.m1 + q.m1, p.m2 + q.m2, p.m3 + q.m3};
+}
+__device__ __forceinline__ PropF pf_sub(const PropF& p, const PropF& q) {
+    return PropF{p.cm1 - q.cm1, p.m1 - q.m1, p.m2 - q.m2, p.m3 - q.m3};
+}
+// coefficient sets kept in an Sp<float> register slot between passes
+__device__ __forceinline__ Sp<float> pf_pack(const PropF& p) { return Sp<float>{p.cm1, p.m1, p.m2, p.m3}; }
+__device__ __forceinline__ PropF pf_unpack(const Sp<float>& v) { return PropF{v.ar, v.ai, v.br, v.bi}; }
 template <class T> __device__ __forceinline__ Sp<T> sp_add(const Sp<T>& a, const Sp<T>& b) {
     return Sp<T>{a.ar + b.ar, a.ai + b.ai, a.br + b.br, a.bi + b.bi};
 }
@@ -570,9 +580,8 @@ template <class ST, class CT, int NS, int GEOM, int CPT, int MODE, int MAXB>
 __global__ void __launch_bounds__(MAXB)
 k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, int max_steps) {
     constexpr bool F64 = std::is_same<CT, double>::value;
-    // mixed precision (FP64 state, FP32 increments): compensated error estimate
-    constexpr bool COMP = !F64 && std::is_same<ST, double>::value;
-    static_assert(!(COMP && MODE == 0), "the mixed stepper recomputes its increments in pass C");
+    // FP32 increments (mixed and FP32 state): compensated error estimate
+    constexpr bool COMP = !F64;
     using R = BRec<GEOM>;
     constexpr int NM = R::NM;
     constexpr int NMV = NM * 3;
@@ -747,12 +756,18 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             src_add<CT>(I, e, s, v1, X0, X2, X3);
                             src_add<CT>(I, e, s, v3, Y0, Y2, Y3);
                         } else {
-                            PropF P1, P0;
-                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], P1);
-                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 0], P0);
+                            PropF P1, P0, PR1, PR0;
+                            prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], P1, PR1);
+                            prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 0], P0, PR0);
                             Sp<float> a = sp_f(C.p0[k][s]);
                             Sp<float> d1 = delta_f(P1, a), d3 = delta_f(P0, a);
-                            if (MODE == 0) { C.k1[k][s] = d1; C.k3[k][s] = d3; }
+                            if (MODE == 0) {
+                                // kept for pass C: the remainder coefficients Q = R0 - R1
+                                // so far, and delta3
+                                const PropF Q{0.f - PR1.cm1, 0.f - PR1.m1, 0.f - PR1.m2, 0.f - PR1.m3};
+                                C.k1[k][s] = pf_pack(pf_add(Q, PR0));
+                                C.k3[k][s] = d3;
+                            }
                             src_add<CT>(I, e, s, sp_add(a, d1), X0, X2, X3);
                             src_add<CT>(I, e, s, sp_add(a, d3), Y0, Y2, Y3);
                         }
@@ -806,13 +821,15 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             src_add<CT>(I, e, s, v4, X0, X2, X3);
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
-                            PropF Pb, P4;
-                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], Pb);
-                            prop_f((float)g0, (float)g1, (float)g2, (float)rec[R::DL + 1], P4);
-                            Sp<float> db = delta_f(Pb, a);
-                            if (MODE == 0) C.k1[k][s] = sp_avg(db, C.k1[k][s]);
+                            PropF Pb, P4, PRb, PR4;
+                            prop_f2((float)g0, (float)g1, (float)g2, (float)rec[R::DL + 1], P4, PR4);
                             Sp<float> d4 = delta_f(P4, a);
-                            if (MODE == 0) C.k4[k][s] = d4;
+                            if (MODE == 0) {
+                                // Q -= Rb, Q += R3 (the order of the recompute path)
+                                prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], Pb, PRb);
+                                C.k1[k][s] = pf_pack(pf_add(pf_sub(pf_unpack(C.k1[k][s]), PRb), PR4));
+                                C.k4[k][s] = d4;
+                            }
                             src_add<CT>(I, e, s, sp_add(a, d4), X0, X2, X3);
                         }
                     }
@@ -898,41 +915,47 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             Sp<float> d1, d3, d4, df;
                             if constexpr (COMP) {
                                 // increments of the state as below; the error from the
-                                // remainders (second order, so rounded relative to x^2)
-                                // plus the first-order part -i Hd psi0 with Hd combined
-                                // per beam in FP64:  psi4 - psi1 = E / 2 with
-                                // E = R0 + R4(psi0 + d3) + L4 d3 + R3 - R1 - Rb - i Hd psi0
+                                // remainders R = (U - I) + i dl H (second order, so
+                                // rounded relative to x^2) plus the first-order part
+                                // -i Hd psi0 with Hd combined per beam in FP64:
+                                //   psi4 - psi1 = E / 2,
+                                //   E = R0 - R1 - Rb + R3 + R4 (psi0 + d3) + L4 d3 - i Hd psi0
+                                //     = (R0 - R1 - Rb + R3 + R4 - i Hd) psi0 + P4 d3
+                                // with R* and P4 = R4 + L4 the coefficient sets of
+                                // (U - I): one application for all remainders, and
+                                // P4 d3 shared with the state's d3n.  Mode 0 kept
+                                // Q = R0 - R1 - Rb + R3, d3 and d4 from passes A and
+                                // B; modes 1/2 recompute them here in the same order.
+                                PropF P, PR, Q;
+                                if (MODE == 0) {
+                                    Q = pf_unpack(C.k1[k][s]); d3 = C.k3[k][s]; d4 = C.k4[k][s];
+                                } else {
                                 double x0, x1, x2;
-                                PropF P, PR;
                                 cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);                    // h0
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
-                                Sp<float> E = delta_f(PR, a);
-                                E = Sp<float>{-E.ar, -E.ai, -E.br, -E.bi};
+                                Q = PropF{0.f - PR.cm1, 0.f - PR.m1, 0.f - PR.m2, 0.f - PR.m3};
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 0], P, PR);
                                 d3 = delta_f(P, a);
-                                E = sp_add(E, delta_f(PR, a));
+                                Q = pf_add(Q, PR);
                                 cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);                    // h1
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
-                                Sp<float> t = delta_f(PR, a);
-                                E = Sp<float>{E.ar - t.ar, E.ai - t.ai, E.br - t.br, E.bi - t.bi};
+                                Q = pf_sub(Q, PR);
                                 cell_h(I, e, s, rec + R::H + 6, x0, x1, x2);                    // h3
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
                                 d4 = delta_f(P, a);
-                                E = sp_add(E, delta_f(PR, a));
+                                Q = pf_add(Q, PR);
+                                }
                                 const float dl2 = (float)rec[R::DL + 2];
-                                const float g0 = (float)h0, g1 = (float)h1, g2 = (float)h2;    // h4
-                                prop_f2(g0, g1, g2, dl2, P, PR);
-                                const Sp<float> b3 = sp_add(a, d3);
-                                const Sp<float> d3n = sp_add(d3, delta_f(P, b3));
-                                E = sp_add(E, delta_f(PR, b3));
-                                const PropF PL{0.f, -g0 * dl2, g2 * dl2, -g1 * dl2};
-                                E = sp_add(E, delta_f(PL, d3));
+                                prop_f2((float)h0, (float)h1, (float)h2, dl2, P, PR);           // h4
+                                const Sp<float> u = delta_f(P, d3);
+                                const Sp<float> d3n = sp_add(d3, sp_add(delta_f(P, a), u));
                                 const double* hd = rec + R::HD;
                                 const double hv0 = I.hv[e], hv1 = I.hv[I.ne + e];
                                 const double sg = (s & 1) ? -1.0 : 1.0;
                                 const PropF PD{0.f, (float)-fma(hd[3], hv0, sg * hd[0]), (float)hd[2],
                                                (float)-fma(hd[3], hv1, sg * hd[1])};
-                                E = sp_add(E, delta_f(PD, a));
+                                Q = pf_add(pf_add(Q, PR), PD);
+                                const Sp<float> E = sp_add(delta_f(Q, a), u);
                                 df = sp_avg(d3n, d4);
                                 dsq = .25 * (double)((E.ar * E.ar + E.ai * E.ai) + (E.br * E.br + E.bi * E.bi));
                             } else {
@@ -981,6 +1004,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // new psi0 at r+dr
                 }
                 // per-beam energy sums of the error (one wave per beam, fixed order)
+                if (tr && tid == 0) tr[14] = (long long)__builtin_amdgcn_s_memrealtime();
                 __syncthreads();
                 for (int b = wave; b < nbt; b += nwaves) {
                     double sq[4] = {0., 0., 0., 0.};
@@ -998,6 +1022,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     double m = fmax(fmax(s_errs[b * 4 + 0], s_errs[b * 4 + 1]), fmax(s_errs[b * 4 + 2], s_errs[b * 4 + 3]));
                     blk_err = fmax(blk_err, m);
                 }
+                if (tr && tid == 0) tr[15] = (long long)__builtin_amdgcn_s_memrealtime();
             }
             acc[NMV] = blk_err;
             ok = reduce_publish<NMV, 1>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
