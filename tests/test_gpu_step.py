@@ -299,3 +299,72 @@ def test_bad_launch_shapes_are_rejected():
         ctx.set_launch(2, 1, 64, 1)        # valid: 40 cells in one wave
         info, _ = ctx.run(5)
         assert info.steps == 5
+
+
+def test_maa_azimuthal_perturbation_matches_oracle():
+    """MAA with an azimuthal-symmetry-breaking seed (nonzero J_x, J_y moments,
+    Phy_MAA.cpp:468-562), reduced from 256 x 64 x 200: FP64 against the oracle,
+    FP32 state against FP64."""
+    capi = _capi()
+    from oracle.oracle import Oracle
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import azimuthal_psi
+    p = bulb_params(Abins=16, Pbins=16, Ebins=24, R0=10, Rn=400, Lve="1.25E+51")
+    psi0 = azimuthal_psi(16, 16, 24, seed=3, amp=1e-7)
+    o = Oracle(p)
+    o.set_state(psi0, 10.0, 1e-3)
+    n, orec, _ = o.run(120)
+    opsi, orr, _, _ = o.state()
+    out = {}
+    for name, pr in (("fp64", capi.XF_FP64), ("fp32", capi.XF_FP32)):
+        with capi.Context(p, precision=pr) as ctx:
+            ctx.upload(psi0, 10.0, 1e-3)
+            rec = _run_all(ctx, 120)
+            out[name] = (rec, *ctx.download())
+    rec, psi, r, _ = out["fp64"]
+    assert np.array_equal(rec["accepted"], orec["accepted"])
+    assert abs(r - orr) <= 1e-12 * orr
+    (pe, pb), (oe, ob) = _pee(psi), _pee(opsi)
+    assert np.abs(pe - oe).max() <= PEE_TOL and np.abs(pb - ob).max() <= PEE_TOL
+    # the seed breaks the azimuthal symmetry: P_ee varies with phi
+    assert np.ptp(pe.reshape(16, 16, 24), axis=1).max() > 0
+    (fe, fb) = _pee(out["fp32"][1])
+    assert np.abs(fe - pe).max() <= 1e-5 and np.abs(fb - pb).max() <= 1e-5
+
+
+def test_seeded_ensemble_fp64_matches_oracle_and_fp32_close():
+    """Instances drawn by the batched-ensemble sampler (uneven grids, either
+    hierarchy, scaled luminosities and densities), reduced in size: every FP64
+    instance against its own oracle run, every FP32-state instance within
+    1e-5 of FP64, with the launch shape chosen by the ensemble autotuner."""
+    capi = _capi()
+    from oracle.oracle import Oracle
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import ensemble_configs, seeded_psi
+    cfgs = ensemble_configs(6, seed=9, abins=(16, 48), ebins=(8, 40))
+    ps = [bulb_params(**dict(d, R0=12, Rn=1000)) for d in cfgs]
+    inits = [seeded_psi(p.abins, p.ebins, seed=50 + i, amp=1e-6) for i, p in enumerate(ps)]
+    res = {}
+    for name, pr in (("fp64", capi.XF_FP64), ("fp32", capi.XF_FP32)):
+        ens = capi.Ensemble(ps, precision=pr)
+        for i, p in enumerate(ps):
+            ens.upload(i, inits[i], p.R0, 1e-3)
+        ens.autotune(4)
+        for i, p in enumerate(ps):
+            ens.upload(i, inits[i], p.R0, 1e-3)
+        infos = ens.run(60)
+        res[name] = (infos, [ens.download(i) for i in range(len(ps))])
+        ens.close()
+    for i, p in enumerate(ps):
+        o = Oracle(p)
+        o.set_state(inits[i], p.R0, 1e-3)
+        o.run(60)
+        opsi, orr, _, (tot, acc) = o.state()
+        info = res["fp64"][0][i]
+        psi, r, _ = res["fp64"][1][i]
+        assert info.total_steps == tot and info.total_accepted == acc, i
+        assert abs(r - orr) <= 1e-12 * orr
+        (pe, pb), (oe, ob) = _pee(psi), _pee(opsi)
+        assert np.abs(pe - oe).max() <= PEE_TOL and np.abs(pb - ob).max() <= PEE_TOL, i
+        (fe, fb) = _pee(res["fp32"][1][i][0])
+        assert np.abs(fe - pe).max() <= 1e-5 and np.abs(fb - pb).max() <= 1e-5, i

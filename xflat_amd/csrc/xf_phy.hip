@@ -346,6 +346,7 @@ bool bad_set(int k) { return k < 0 || k > 3; }
 template <int KIND>
 int launch_nbeam(XfPhy* p, const double* in, double* out, double* avg, int pt, const double* hvv, double hm,
                  double* hsum, double* eps) {
+    (void)hipGetLastError();    // a launch error, not one left by an earlier handled call
     hipLaunchKernelGGL(k_nbeam<KIND>, dim3(p->d.nbeam * 4), dim3(256), 0, 0, p->d, in, out, avg, pt, hvv, hm, hsum, eps);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
@@ -426,6 +427,7 @@ int xf_calc_angle_bins(xf_ctx* ctx, double r, int pt) {
     int rc = get_phy(ctx, &p);
     if (rc) return rc;
     if (pt < 0 || pt > 2) return xf_fail(XF_EINVAL, "bad point");
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_angle_bins, dim3((p->d.nth + 255) / 256), dim3(256), 0, 0, p->d, r, pt);
     HIPCHK(hipGetLastError());
     return XF_OK;
@@ -436,6 +438,7 @@ int xf_calc_delta_ls(xf_ctx* ctx, double dr, int cur, int s, int e) {
     int rc = get_phy(ctx, &p);
     if (rc) return rc;
     if (cur < 0 || cur > 2 || s < 0 || s > 2 || e < 0 || e > 2) return xf_fail(XF_EINVAL, "bad point");
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_delta_ls, dim3((p->d.nth + 255) / 256), dim3(256), 0, 0, p->d, dr, cur, s, e);
     HIPCHK(hipGetLastError());
     return XF_OK;
@@ -446,6 +449,7 @@ int xf_calc_hvv(xf_ctx* ctx, int set, int pt, int hvv) {
     int rc = get_phy(ctx, &p);
     if (rc) return rc;
     if (bad_set(set) || bad_set(hvv) || pt < 0 || pt > 2) return xf_fail(XF_EINVAL, "bad arguments");
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_hvv_partial, dim3((p->d.nth + 63) / 64), dim3(64), 0, 0, p->d, p->hsum[set], pt, p->part);
     hipLaunchKernelGGL(k_hvv_final, dim3(1), dim3(256), 0, 0, p->d, p->part, pt, p->hvv[hvv]);
     HIPCHK(hipGetLastError());

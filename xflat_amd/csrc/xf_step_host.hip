@@ -4,6 +4,8 @@
 #include <algorithm>
 #include "xf_internal.h"
 
+constexpr size_t XF_LDS_MAX = 160 * 1024;        // LDS per workgroup on gfx950
+
 using XfStepFn = void (*)(const XfInstDev*, const int*, int);
 XfStepFn xf_pick_fp64_ma(int, int, int);
 XfStepFn xf_pick_fp64_maa(int, int, int);
@@ -41,6 +43,7 @@ int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {
     int ncu = 0;
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     size_t smem = xf_step_smem_bytes(L, geom);
+    if (smem > XF_LDS_MAX) return xf_fail(XF_EINVAL, "shape needs %zu B of LDS, the workgroup limit is %zu", smem, XF_LDS_MAX);
     if (smem > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
@@ -68,6 +71,7 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
     XfStepFn f = pick(L, geom);
     if (!f) return xf_fail(XF_EINVAL, "no stepper for this shape");
     size_t smem = xf_step_smem_bytes(L, geom);
+    if (smem > XF_LDS_MAX) return xf_fail(XF_EINVAL, "shape needs %zu B of LDS, the workgroup limit is %zu", smem, XF_LDS_MAX);
     if (smem > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
@@ -91,6 +95,7 @@ __global__ void k_step_prepare(const XfInstDev* __restrict__ insts) {
 
 int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream) {
     if (ninst <= 0 || ninst > 65535) return xf_fail(XF_EINVAL, "%d instances in one launch", ninst);
+    (void)hipGetLastError();    // a launch error, not one left by an earlier handled call
     hipLaunchKernelGGL(k_step_prepare, dim3(8, ninst), dim3(256), 0, (hipStream_t)stream, d_insts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return xf_fail(XF_EHIP, "prepare launch: %s", hipGetErrorString(e));

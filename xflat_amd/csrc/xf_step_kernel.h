@@ -132,33 +132,38 @@ __device__ __forceinline__ void prop_f(float h0, float h1, float h2, float dl, P
     P.m3 = -h1 * sc;
 }
 // prop_f plus the coefficients of the remainder R = (U - I) + i dl H, i.e.
-// (cos x - 1) I - i (sin(x)/lambda - dl) H: second order in x = lambda dl, its
-// sin(x)/lambda - dl = dl (sin(x)/x - 1) from the series -x^2/6 + x^4/120 -
-// x^6/5040 for x < 1/4 (next term < 5e-9 relative), so its FP32 rounding is
-// relative to x^2, not to x.
+// (cos x - 1) I - i (sin(x)/lambda - dl) H, second order in x = lambda dl.
+// For x^2 < 1/16 all coefficients come from their Taylor series in x^2
+// (cos x - 1 = -x^2/2 + x^4/24 - x^6/720 + x^8/40320, sin(x)/x - 1 =
+// -x^2/6 + x^4/120 - x^6/5040 + x^8/362880; the next terms are below 3e-13
+// relative): each is rounded relative to its own size, the remainders to x^2
+// rather than to x, and no square root or transcendental is needed.  Larger
+// angles use the hardware sin/cos of the half angle.
 __device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, PropF& P, PropF& PR) {
-    float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
-    float rl = lam2 > 0.f ? __builtin_amdgcn_rsqf(lam2) : 0.f;
-    float lam = lam2 * rl;
-    float xr = lam * dl;
-    float x = xr * (float)(0.5 * INV_2PI_D);                // half angle, revolutions
-    x = x - __builtin_rintf(x);
-    float sh = __builtin_amdgcn_sinf(x);
-    float ch = __builtin_amdgcn_cosf(x);
-    float s = 2.f * sh * ch;
-    float sc = s * rl;
-    float scr;
-    if (xr < 0.25f) {
-        float x2 = xr * xr;
-        scr = dl * x2 * fmaf(x2, fmaf(x2, -1.f / 5040.f, 1.f / 120.f), -1.f / 6.f);
+    const float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
+    const float x2 = lam2 * (dl * dl);
+    float cm1, sc, scr;
+    if (x2 < 0.0625f) {
+        cm1 = x2 * fmaf(x2, fmaf(x2, fmaf(x2, 1.f / 40320.f, -1.f / 720.f), 1.f / 24.f), -0.5f);
+        const float S = x2 * fmaf(x2, fmaf(x2, fmaf(x2, 1.f / 362880.f, -1.f / 5040.f), 1.f / 120.f), -1.f / 6.f);
+        scr = dl * S;
+        sc = dl + scr;
     } else {
+        const float rl = __builtin_amdgcn_rsqf(lam2);
+        const float xr = lam2 * rl * dl;
+        float x = xr * (float)(0.5 * INV_2PI_D);            // half angle, revolutions
+        x = x - __builtin_rintf(x);
+        const float sh = __builtin_amdgcn_sinf(x);
+        const float ch = __builtin_amdgcn_cosf(x);
+        cm1 = -2.f * sh * sh;
+        sc = 2.f * sh * ch * rl;
         scr = sc - dl;
     }
-    P.cm1 = -2.f * sh * sh;
+    P.cm1 = cm1;
     P.m1 = -h0 * sc;
     P.m2 = h2 * sc;
     P.m3 = -h1 * sc;
-    PR.cm1 = P.cm1;
+    PR.cm1 = cm1;
     PR.m1 = -h0 * scr;
     PR.m2 = h2 * scr;
     PR.m3 = -h1 * scr;
@@ -402,6 +407,22 @@ __device__ __forceinline__ void src_add(const XfInstDev& I, int e, int s, const 
     if (s & 1) { X0 -= w * d0; X2 -= w * d2; }
     else       { X0 += w * d0; X2 += w * d2; }
     X3 += w * d3;
+}
+
+// Source contribution of the increment d of a stage a + d: w (rho(a + d) -
+// rho(a)) from the first-order expansion (no rounding of a + d), so stages of
+// one step differ only by rounding relative to their increments and share
+// the rounding of rho(a), which the caller accumulates once per stage.
+__device__ __forceinline__ void src_dadd(const XfInstDev& I, int e, int s, const Sp<float>& a, const Sp<float>& d,
+                                         float& X0, float& X2, float& X3) {
+    const float t0 = ((a.ar + .5f * d.ar) * d.ar + (a.ai + .5f * d.ai) * d.ai) -
+                     ((a.br + .5f * d.br) * d.br + (a.bi + .5f * d.bi) * d.bi);
+    const float t2 = (a.ar * d.br + d.ar * (a.br + d.br)) + (a.ai * d.bi + d.ai * (a.bi + d.bi));
+    const float t3 = (a.ai * d.br + d.ai * (a.br + d.br)) - (a.ar * d.bi + d.ar * (a.bi + d.bi));
+    const float w = I.srcwf[s * I.ne + e];
+    if (s & 1) { X0 -= w * t0; X2 -= w * t2; }
+    else       { X0 += w * t0; X2 += w * t2; }
+    X3 += w * t3;
 }
 
 template <int NM>
@@ -741,6 +762,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     const double* rec = s_beam + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0), Y0 = CT(0), Y2 = CT(0), Y3 = CT(0);
+                    CT Z0 = CT(0), Z2 = CT(0), Z3 = CT(0);     // FP32 increments: the shared rho(psi0) part
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         double h0, h1, h2;
@@ -768,12 +790,17 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 C.k1[k][s] = pf_pack(pf_add(Q, PR0));
                                 C.k3[k][s] = d3;
                             }
-                            src_add<CT>(I, e, s, sp_add(a, d1), X0, X2, X3);
-                            src_add<CT>(I, e, s, sp_add(a, d3), Y0, Y2, Y3);
+                            src_add<CT>(I, e, s, a, Z0, Z2, Z3);
+                            src_dadd(I, e, s, a, d1, X0, X2, X3);
+                            src_dadd(I, e, s, a, d3, Y0, Y2, Y3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);        // h1 at r+dr
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
+                    if constexpr (!F64) {
+                        acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)Z0, (double)Z2, (double)Z3);
+                        acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Z0, (double)Z2, (double)Z3);
+                    }
                 }
             }
             ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
@@ -804,6 +831,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     const double* rec = s_beam + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
+                    CT Z0 = CT(0), Z2 = CT(0), Z3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         double h0, h1, h2, g0, g1, g2;
@@ -830,10 +858,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 C.k1[k][s] = pf_pack(pf_add(pf_sub(pf_unpack(C.k1[k][s]), PRb), PR4));
                                 C.k4[k][s] = d4;
                             }
-                            src_add<CT>(I, e, s, sp_add(a, d4), X0, X2, X3);
+                            src_add<CT>(I, e, s, a, Z0, Z2, Z3);
+                            src_dadd(I, e, s, a, d4, X0, X2, X3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
+                    if constexpr (!F64) acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)Z0, (double)Z2, (double)Z3);
                 }
             }
             ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
