@@ -133,19 +133,27 @@ __device__ __forceinline__ void prop_f(float h0, float h1, float h2, float dl, P
 }
 // prop_f plus the coefficients of the remainder R = (U - I) + i dl H, i.e.
 // (cos x - 1) I - i (sin(x)/lambda - dl) H, second order in x = lambda dl.
-// For x^2 < 1/16 all coefficients come from their Taylor series in x^2
-// (cos x - 1 = -x^2/2 + x^4/24 - x^6/720 + x^8/40320, sin(x)/x - 1 =
-// -x^2/6 + x^4/120 - x^6/5040 + x^8/362880; the next terms are below 3e-13
-// relative): each is rounded relative to its own size, the remainders to x^2
-// rather than to x, and no square root or transcendental is needed.  Larger
-// angles use the hardware sin/cos of the half angle.
+// For x^2 < 1 all coefficients come from polynomials in x^2
+// (cos x - 1 = -x^2/2 + x^4/24 - ..., sin(x)/x - 1 = -x^2/6 + x^4/120 - ...):
+// each is rounded relative to its own size, the remainders to x^2 rather than
+// to x, and no square root or transcendental is needed.  Larger angles take
+// the hardware sin/cos of the half angle.
 __device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, PropF& P, PropF& PR) {
     const float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
     const float x2 = lam2 * (dl * dl);
     float cm1, sc, scr;
-    if (x2 < 0.0625f) {
-        cm1 = x2 * fmaf(x2, fmaf(x2, fmaf(x2, 1.f / 40320.f, -1.f / 720.f), 1.f / 24.f), -0.5f);
-        const float S = x2 * fmaf(x2, fmaf(x2, fmaf(x2, 1.f / 362880.f, -1.f / 5040.f), 1.f / 120.f), -1.f / 6.f);
+    if (x2 < 1.f) {
+        // (cos x - 1)/t and (sin x / x - 1)/t as cubics in t = x^2: the
+        // degree-3 Chebyshev interpolants on t in [0, 1] of their Taylor
+        // series (Chebyshev economization), coefficients rounded to FP32.
+        // Evaluated by FMA Horner they stay within 5e-8 and 8.2e-8 relative
+        // of the exact functions over the whole range, i.e. at FP32 rounding,
+        // so the remainder coefficients (scr) keep their accuracy, which the
+        // hardware sin/cos path cannot give after the cancellation sc - dl.
+        const float pc = fmaf(x2, fmaf(x2, fmaf(x2, 2.42562728e-05f, -0.00138854922f), 0.0416665971f), -0.5f);
+        const float ps = fmaf(x2, fmaf(x2, fmaf(x2, 2.70607666e-06f, -0.000198381764f), 0.00833332725f), -0.166666672f);
+        cm1 = x2 * pc;
+        const float S = x2 * ps;
         scr = dl * S;
         sc = dl + scr;
     } else {
@@ -407,22 +415,6 @@ __device__ __forceinline__ void src_add(const XfInstDev& I, int e, int s, const 
     if (s & 1) { X0 -= w * d0; X2 -= w * d2; }
     else       { X0 += w * d0; X2 += w * d2; }
     X3 += w * d3;
-}
-
-// Source contribution of the increment d of a stage a + d: w (rho(a + d) -
-// rho(a)) from the first-order expansion (no rounding of a + d), so stages of
-// one step differ only by rounding relative to their increments and share
-// the rounding of rho(a), which the caller accumulates once per stage.
-__device__ __forceinline__ void src_dadd(const XfInstDev& I, int e, int s, const Sp<float>& a, const Sp<float>& d,
-                                         float& X0, float& X2, float& X3) {
-    const float t0 = ((a.ar + .5f * d.ar) * d.ar + (a.ai + .5f * d.ai) * d.ai) -
-                     ((a.br + .5f * d.br) * d.br + (a.bi + .5f * d.bi) * d.bi);
-    const float t2 = (a.ar * d.br + d.ar * (a.br + d.br)) + (a.ai * d.bi + d.ai * (a.bi + d.bi));
-    const float t3 = (a.ai * d.br + d.ai * (a.br + d.br)) - (a.ar * d.bi + d.ar * (a.bi + d.bi));
-    const float w = I.srcwf[s * I.ne + e];
-    if (s & 1) { X0 -= w * t0; X2 -= w * t2; }
-    else       { X0 += w * t0; X2 += w * t2; }
-    X3 += w * t3;
 }
 
 template <int NM>
@@ -762,7 +754,6 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     const double* rec = s_beam + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0), Y0 = CT(0), Y2 = CT(0), Y3 = CT(0);
-                    CT Z0 = CT(0), Z2 = CT(0), Z3 = CT(0);     // FP32 increments: the shared rho(psi0) part
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         double h0, h1, h2;
@@ -790,17 +781,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 C.k1[k][s] = pf_pack(pf_add(Q, PR0));
                                 C.k3[k][s] = d3;
                             }
-                            src_add<CT>(I, e, s, a, Z0, Z2, Z3);
-                            src_dadd(I, e, s, a, d1, X0, X2, X3);
-                            src_dadd(I, e, s, a, d3, Y0, Y2, Y3);
+                            src_add<CT>(I, e, s, sp_add(a, d1), X0, X2, X3);
+                            src_add<CT>(I, e, s, sp_add(a, d3), Y0, Y2, Y3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);        // h1 at r+dr
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
-                    if constexpr (!F64) {
-                        acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)Z0, (double)Z2, (double)Z3);
-                        acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Z0, (double)Z2, (double)Z3);
-                    }
                 }
             }
             ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
@@ -831,7 +817,6 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     const double* rec = s_beam + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
-                    CT Z0 = CT(0), Z2 = CT(0), Z3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         double h0, h1, h2, g0, g1, g2;
@@ -858,12 +843,10 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 C.k1[k][s] = pf_pack(pf_add(pf_sub(pf_unpack(C.k1[k][s]), PRb), PR4));
                                 C.k4[k][s] = d4;
                             }
-                            src_add<CT>(I, e, s, a, Z0, Z2, Z3);
-                            src_dadd(I, e, s, a, d4, X0, X2, X3);
+                            src_add<CT>(I, e, s, sp_add(a, d4), X0, X2, X3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
-                    if constexpr (!F64) acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)Z0, (double)Z2, (double)Z3);
                 }
             }
             ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
