@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "beam-steps/sec (angle x energy x dr) at stated P_ee error vs FP64; % HBM roofline"
 R0, DR0 = 10.0, 1e-3
 PROFILE_HBM = os.path.join(ROOT, "profiles", "k_step_fp32_ma_1000x400_hbm.md")
-PROFILED = ("fp32", 0, 4, 4)         # precision, mode, cells per lane, beams per tile of that profile
+PROFILED = ("fp32", 1, 4, 4)         # precision, mode, cells per lane, beams per tile of that profile
 
 WORKLOADS = {
     "ma1000x400": dict(
