@@ -116,13 +116,13 @@ struct XfInst {
         return XF_OK;
     }
 
-    // exchange rows for nblk workgroups: [2][XF_ROWW][nblk] words
+    // exchange rows for nblk workgroups: [2][XF_ROW_REPLICAS][XF_ROWW][nblk] words
     int ensure_rows(int nblk) {
         if (nblk <= rows_nblk) return XF_OK;
         if (d_rows) (void)hipFree(d_rows);
         d_rows = nullptr;
         rows_nblk = 0;
-        int rc = dalloc(&d_rows, (size_t)2 * XF_ROWW * nblk);
+        int rc = dalloc(&d_rows, (size_t)2 * XF_ROW_REPLICAS * XF_ROWW * nblk);
         if (rc) return rc;
         rows_nblk = nblk;
         dev.rows = d_rows;

@@ -66,6 +66,7 @@ struct XfRunScalars {
 // restart at 0); a launch performs at most XF_MAX_EXCHANGES exchanges.
 constexpr int XF_NVMAX = 32;                   // max values per partial row
 constexpr int XF_ROWW = 2 * XF_NVMAX;          // words per row
+constexpr int XF_ROW_REPLICAS = 8;             // copies of every row (reads spread over more lines)
 constexpr long long XF_MAX_EXCHANGES = 0xF0000000ll;
 
 struct XfInstDev {
@@ -94,7 +95,7 @@ struct XfInstDev {
     // state and work buffers
     void* psi[2];                  // [ns*4][ncells] planes, double or float
     XfRunScalars* rs;
-    unsigned long long* rows;      // [2][XF_ROWW][nblk] tagged partial rows
+    unsigned long long* rows;      // [2][XF_ROW_REPLICAS][XF_ROWW][nblk] tagged partial rows
     xf_step_rec* log;             // optional per-attempt log
     long long* trace;              // optional [steps][16] real-time stamps of workgroup 0
     int trace_steps;

@@ -28,10 +28,10 @@ bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nu
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
     int nm = geom == XF_MAA ? 4 : 2;
     int bsize = 3 + 3 * nm + 9 + 12 + 4;
-    int nwaves = L.block / 64;
-    // the pass-C error buffer doubles as the exchange staging [2 * 3 * nm][xrows]
+    // the pass-C error buffer doubles as the exchange staging: the lanes'
+    // values (in chunks of values) and the fetched rows [2 * 3 * nm][xrows]
     size_t errb = std::max((size_t)4 * L.block * L.cpt, (size_t)6 * nm * std::max(L.xrows, 1));
-    return sizeof(double) * ((size_t)L.bpt * bsize + errb + 4 * (size_t)L.bpt + (size_t)nwaves * 8 * XF_NVMAX + XF_NVMAX);
+    return sizeof(double) * ((size_t)L.bpt * bsize + errb + 4 * (size_t)L.bpt + XF_NVMAX);
 }
 
 int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {
@@ -88,7 +88,7 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
 __global__ void k_step_prepare(const XfInstDev* __restrict__ insts) {
     const XfInstDev& I = insts[blockIdx.y];
     if (blockIdx.x == 0 && threadIdx.x == 0) I.rs->status = 0;
-    const long long n = 2ll * XF_ROWW * I.nblk;
+    const long long n = 2ll * XF_ROW_REPLICAS * XF_ROWW * I.nblk;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         I.rows[i] = 0ull;
 }

@@ -49,7 +49,7 @@ namespace {
 constexpr double INV_2PI_D = 0.15915494309189535;
 constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;   // 2 s at 100 MHz
 #ifndef XF_EXCHANGE_WORDS
-#define XF_EXCHANGE_WORDS 8      // words per thread in flight in the exchange fetch
+#define XF_EXCHANGE_WORDS 16     // words per thread in flight in the exchange fetch
 #endif
 
 template <class T> struct Sp { T ar, ai, br, bi; };
@@ -446,73 +446,103 @@ __device__ __forceinline__ Sp<double> candidate(const Sp<double>& p0, const Sp<f
     return nd;
 }
 
-// Workgroup reduction of NV sums and NMAX maxima held one set per lane ->
-// s_tot.  Fixed structure (a butterfly over lane bits 5..3, then 16 lanes per
-// value over the per-wave partials in LDS), so equal inputs give equal bits.
-template <int NV, int NMAX>
-__device__ __forceinline__ void block_reduce(const double* acc, double* s_red, double* s_tot) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-    constexpr int NT = NV + NMAX;
-#pragma unroll
-    for (int v = 0; v < NT; ++v) {
-        double x = acc[v];
-#pragma unroll
-        for (int o = 32; o >= 8; o >>= 1) {
-            double y = __shfl_xor(x, o, 64);
-            x = (v < NV) ? x + y : fmax(x, y);
-        }
-        if (lane < 8) s_red[(v * nwaves + wave) * 8 + lane] = x;
-    }
-    __syncthreads();
-    for (int t0 = 0; t0 < NT * 16; t0 += blockDim.x) {
-        const int t = t0 + tid;
-        const int v = t >> 4, j = t & 15;
-        double x = 0.;
-        if (v < NT) {
-            const double* src = s_red + v * nwaves * 8;
-            for (int i = j; i < nwaves * 8; i += 16) x = (v < NV) ? x + src[i] : fmax(x, src[i]);
-        }
-#pragma unroll
-        for (int o = 8; o >= 1; o >>= 1) {
-            double y = __shfl_xor(x, o, 16);
-            x = (v < NV) ? x + y : fmax(x, y);
-        }
-        if (v < NT && j == 0) s_tot[v] = x;
-    }
-    __syncthreads();
+// Wave reductions on the DPP network (VALU moves, no LDS round trip), one
+// 32-bit half of a double at a time.  wave_red_dpp leaves the total of all
+// 64 lanes in lane 63: rotations by 8, 4, 2, 1 within each row of 16 lanes
+// give every lane its row's total, then row 0's and row 2's totals (lanes 15
+// and 47) are broadcast into rows 1 and 3, and lane 31's into rows 2 and 3.
+// The association order is fixed, so equal inputs give equal bits.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <bool MAX>
+__device__ __forceinline__ double dpp_op(double a, double b) { return MAX ? fmax(a, b) : a + b; }
+// MAX: maxima of values >= 0 (lanes outside a broadcast's row mask read 0).
+template <bool MAX>
+__device__ __forceinline__ double wave_red_dpp(double v) {
+    v = dpp_op<MAX>(v, dpp_d<0x128, 0xF>(v));    // row_ror:8
+    v = dpp_op<MAX>(v, dpp_d<0x124, 0xF>(v));    // row_ror:4
+    v = dpp_op<MAX>(v, dpp_d<0x122, 0xF>(v));    // row_ror:2
+    v = dpp_op<MAX>(v, dpp_d<0x121, 0xF>(v));    // row_ror:1
+    v = dpp_op<MAX>(v, dpp_d<0x142, 0xA>(v));    // row_bcast:15 into rows 1, 3
+    v = dpp_op<MAX>(v, dpp_d<0x143, 0xC>(v));    // row_bcast:31 into rows 2, 3
+    return v;
+}
+// lane 63's value in every lane
+__device__ __forceinline__ double lane63(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 // Cross-workgroup reduction of NV sums and NMAX maxima (maxima of values
 // >= 0) by an all-gather of tagged rows:
-//  1. the workgroup reduces its lanes (fixed structure) and publishes the
-//     result as row lb: value v -> words 2v (low half) and 2v + 1 (high
-//     half), each tagged with gen + 1, in rows[gen & 1][word][nblk];
-//  2. wave 0 polls word 0 of all rows (one contiguous array) until every
-//     row carries the tag;
-//  3. the workgroup fetches all words (coalesced), checks every tag, stages
-//     the payloads in LDS and sums each value over the rows in a fixed
-//     order, which leaves the same bits in every workgroup.
-// The rows are the only data that crosses workgroups, so no fence is needed:
-// each word is one single-copy-atomic 64-bit store and its tag names its
-// exchange.  Two parities suffice because a workgroup publishes exchange
-// gen + 2 only after every row of gen + 1 arrived, and a workgroup publishes
-// gen + 1 only after its fetch of gen returned (the published values depend
-// on the fetched ones).  All waits are bounded.
+//  1. the lanes' values are staged in LDS (xcap doubles, in chunks of
+//     values when NT * B exceeds it); wave w reduces values w, w + nwaves,
+//     ...: lane l sums lanes l, l + 64, ... in order, then the DPP wave
+//     reduction, and the wave publishes the total as row lb: value v ->
+//     words 2v (low half) and 2v + 1 (high half), each tagged with gen + 1,
+//     into every replica of rows[gen & 1][replica][word][nblk];
+//  2. wave 0 polls word 0 of all rows of replica lb % XF_ROW_REPLICAS (one
+//     contiguous array) until every row carries the tag;
+//  3. the workgroup fetches all words of that replica (coalesced, 16 words
+//     per lane in flight), checks every tag, stages the payloads in LDS and
+//     sums each value over the rows in a fixed order (one wave per value,
+//     lane l summing rows l, l + 64, ..., then the DPP reduction), which
+//     leaves the same bits in every workgroup.
+// Replicas spread the all-gather's reads, which every workgroup makes of the
+// same few kilobytes, over more memory lines.  The rows are the only data
+// that crosses workgroups, so no fence is needed: each word is one
+// single-copy-atomic 64-bit store and its tag names its exchange.  Two
+// parities suffice because a workgroup publishes exchange gen + 2 only after
+// every row of gen + 1 arrived, and a workgroup publishes gen + 1 only after
+// its fetch of gen returned (the published values depend on the fetched
+// ones).  All waits are bounded.
 template <int NV, int NMAX>
-__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red, double* s_tot, double* s_x, const XfInstDev& I,
+__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot, double* s_x, int xcap, const XfInstDev& I,
                                                int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr) {
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int B = blockDim.x;
     const int nblk = I.nblk;
     constexpr int NT = NV + NMAX;
+    constexpr int RP = XF_ROW_REPLICAS;
     const unsigned tag = gen + 1u;
     const unsigned long long tagw = (unsigned long long)tag << 32;
-    unsigned long long* rows = I.rows + (size_t)(gen & 1u) * XF_ROWW * nblk;
+    const size_t rep_words = (size_t)XF_ROWW * nblk;
+    unsigned long long* rows0 = I.rows + (size_t)(gen & 1u) * RP * rep_words;
+    unsigned long long* rows = rows0 + (size_t)(lb % RP) * rep_words;
     if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
-    block_reduce<NV, NMAX>(acc, s_red, s_tot);
-    if (tid < 2 * NT) {
-        const unsigned long long u = (unsigned long long)__double_as_longlong(s_tot[tid >> 1]);
-        const unsigned half = (tid & 1) ? (unsigned)(u >> 32) : (unsigned)u;
-        st_word(rows + (size_t)tid * nblk + lb, tagw | half);
+    const int vg = max(1, min(NT, xcap / B));          // values staged at once
+    for (int v0 = 0; v0 < NT; v0 += vg) {
+        const int nv = min(vg, NT - v0);
+        if (v0 > 0) __syncthreads();                   // the previous chunk has been read
+#pragma unroll
+        for (int v = 0; v < NT; ++v)
+            if (v >= v0 && v < v0 + nv) s_x[(v - v0) * B + tid] = acc[v];
+        __syncthreads();
+        for (int c = wave; c < nv; c += nwaves) {
+            const int v = v0 + c;
+            const double* src = s_x + (size_t)c * B;
+            double x = 0.;
+            if (v < NV) {
+                for (int i = lane; i < B; i += 64) x += src[i];
+                x = lane63(wave_red_dpp<false>(x));
+            } else {
+                for (int i = lane; i < B; i += 64) x = fmax(x, src[i]);
+                x = lane63(wave_red_dpp<true>(x));
+            }
+            if (lane >= 64 - 2 * RP) {
+                const int q = 63 - lane;
+                const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+                const unsigned half = (q & 1) ? (unsigned)(u >> 32) : (unsigned)u;
+                st_word(rows0 + (size_t)(q >> 1) * rep_words + (size_t)(2 * v + (q & 1)) * nblk + lb, tagw | half);
+            }
+        }
     }
     if (tr && tid == 0) tr[1] = (long long)__builtin_amdgcn_s_memrealtime();
     if (tid < 64) {
@@ -535,7 +565,6 @@ __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red,
     // the whole workgroup fetch again.
     constexpr int XW = XF_EXCHANGE_WORDS;
     const int nw = 2 * NT * nblk;
-    const int B = blockDim.x;
     unsigned* s_h = reinterpret_cast<unsigned*>(s_x);
     const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
     for (int spins = 1; !fail; ++spins) {
@@ -558,26 +587,18 @@ __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_red,
         fail = __syncthreads_or(wait_expired(t1, 16 * spins, status));
     }
     if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
-    // per value: 16 lanes sum rows j = l, l + 16, ... in order, then a
-    // butterfly over the 16 lanes -- the same bits in every workgroup
-    for (int t0 = 0; t0 < NT * 16; t0 += B) {
-        const int t = t0 + tid;
-        const int v = t >> 4, l = t & 15;
+    for (int v = wave; v < NT; v += nwaves) {
         double x = 0.;
-        if (v < NT && !fail) {
+        if (!fail) {
             const unsigned* lo = s_h + (size_t)(2 * v) * nblk;
             const unsigned* hi = lo + nblk;
-            for (int j = l; j < nblk; j += 16) {
+            for (int j = lane; j < nblk; j += 64) {
                 const double y = __longlong_as_double((long long)(((unsigned long long)hi[j] << 32) | lo[j]));
                 x = (v < NV) ? x + y : fmax(x, y);
             }
         }
-#pragma unroll
-        for (int o = 8; o >= 1; o >>= 1) {
-            const double y = __shfl_xor(x, o, 16);
-            x = (v < NV) ? x + y : fmax(x, y);
-        }
-        if (v < NT && l == 0) s_tot[v] = x;
+        x = (v < NV) ? wave_red_dpp<false>(x) : wave_red_dpp<true>(x);
+        if (lane == 63) s_tot[v] = x;
     }
     __syncthreads();
     gen += 1u;
@@ -619,9 +640,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     double* s_beam = smem;                                   // [bpt][R::SIZE]
     double* s_errb = s_beam + bpt * R::SIZE;                 // [4][B*CPT], between passes the
                                                              // exchange's staging [2*NMV][nblk]
-    double* s_errs = s_errb + max(4 * B * CPT, 2 * NMV * nblk);   // [bpt][4]
-    double* s_red = s_errs + 4 * bpt;                        // [XF_NVMAX][nwaves][8]
-    double* s_tot = s_red + nwaves * 8 * XF_NVMAX;           // [XF_NVMAX]
+    const int xcap = max(4 * B * CPT, 2 * NMV * nblk);     // doubles of s_errb
+    double* s_errs = s_errb + xcap;                          // [bpt][4]
+    double* s_tot = s_errs + 4 * bpt;                        // [XF_NVMAX]
     __shared__ int s_flag[2];
     // angular moments, uniform over the workgroup: psi0, psi1, psi3, psi4, candidate
     __shared__ double s_M0[12], s_M1[12], s_M3[12], s_M4[12], s_Mn[12];
@@ -717,7 +738,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 acc_moments<NM>(acc, rec + R::W + 0 * NM, (double)X0, (double)X2, (double)X3);
             }
         }
-        ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag);
+        ok = reduce_publish<NMV, 0>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag);
         if (tid < NMV) M0[tid] = s_tot[tid];
         __syncthreads();
     }
@@ -789,7 +810,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
                 }
             }
-            ok = reduce_publish<2 * NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
+            ok = reduce_publish<2 * NMV, 0>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
             if (tid < NMV) { M1[tid] = s_tot[tid]; M3[tid] = s_tot[NMV + tid]; }
             __syncthreads();
             if (!ok) break;
@@ -849,7 +870,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
                 }
             }
-            ok = reduce_publish<NMV, 0>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
+            ok = reduce_publish<NMV, 0>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
             if (tid < NMV) M4[tid] = s_tot[tid];
             __syncthreads();
             if (!ok) break;
@@ -1038,7 +1059,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (tr && tid == 0) tr[15] = (long long)__builtin_amdgcn_s_memrealtime();
             }
             acc[NMV] = blk_err;
-            ok = reduce_publish<NMV, 1>(acc, s_red, s_tot, s_errb, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
+            ok = reduce_publish<NMV, 1>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
             if (tid < NMV) Mn[tid] = s_tot[tid];
             errmax = s_tot[NMV];
             __syncthreads();
