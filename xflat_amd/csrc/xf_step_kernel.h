@@ -221,18 +221,6 @@ template <class T> __device__ __forceinline__ T dnorm2(const Sp<T>& x, const Sp<
     return (dar * dar + dai * dai) + (dbr * dbr + dbi * dbi);
 }
 
-// ---------------------------------------------------------------- reductions
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 // Tagged row words of the cross-workgroup exchange: 32-bit generation tag
 // (exchange index + 1) over one 32-bit half of a partial sum.
 __device__ __forceinline__ unsigned long long ld_word(const unsigned long long* p) {
@@ -525,22 +513,31 @@ __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot,
         for (int v = 0; v < NT; ++v)
             if (v >= v0 && v < v0 + nv) s_x[(v - v0) * B + tid] = acc[v];
         __syncthreads();
-        for (int c = wave; c < nv; c += nwaves) {
-            const int v = v0 + c;
-            const double* src = s_x + (size_t)c * B;
-            double x = 0.;
-            if (v < NV) {
-                for (int i = lane; i < B; i += 64) x += src[i];
-                x = lane63(wave_red_dpp<false>(x));
-            } else {
-                for (int i = lane; i < B; i += 64) x = fmax(x, src[i]);
-                x = lane63(wave_red_dpp<true>(x));
+        // two values per wave at a time, so their load and DPP chains interleave
+        for (int c = wave; c < nv; c += 2 * nwaves) {
+            const int cc[2] = {c, c + nwaves};
+            double x[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int v = v0 + cc[h];
+                const double* src = s_x + (size_t)cc[h] * B;
+                x[h] = 0.;
+                if (cc[h] < nv) {
+                    if (v < NV) for (int i = lane; i < B; i += 64) x[h] += src[i];
+                    else        for (int i = lane; i < B; i += 64) x[h] = fmax(x[h], src[i]);
+                }
             }
-            if (lane >= 64 - 2 * RP) {
-                const int q = 63 - lane;
-                const unsigned long long u = (unsigned long long)__double_as_longlong(x);
-                const unsigned half = (q & 1) ? (unsigned)(u >> 32) : (unsigned)u;
-                st_word(rows0 + (size_t)(q >> 1) * rep_words + (size_t)(2 * v + (q & 1)) * nblk + lb, tagw | half);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int v = v0 + cc[h];
+                if (cc[h] >= nv) break;               // wave-uniform
+                x[h] = lane63(v < NV ? wave_red_dpp<false>(x[h]) : wave_red_dpp<true>(x[h]));
+                if (lane >= 64 - 2 * RP) {
+                    const int q = 63 - lane;
+                    const unsigned long long u = (unsigned long long)__double_as_longlong(x[h]);
+                    const unsigned half = (q & 1) ? (unsigned)(u >> 32) : (unsigned)u;
+                    st_word(rows0 + (size_t)(q >> 1) * rep_words + (size_t)(2 * v + (q & 1)) * nblk + lb, tagw | half);
+                }
             }
         }
     }
@@ -587,18 +584,28 @@ __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot,
         fail = __syncthreads_or(wait_expired(t1, 16 * spins, status));
     }
     if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
-    for (int v = wave; v < NT; v += nwaves) {
-        double x = 0.;
-        if (!fail) {
-            const unsigned* lo = s_h + (size_t)(2 * v) * nblk;
-            const unsigned* hi = lo + nblk;
-            for (int j = lane; j < nblk; j += 64) {
-                const double y = __longlong_as_double((long long)(((unsigned long long)hi[j] << 32) | lo[j]));
-                x = (v < NV) ? x + y : fmax(x, y);
+    for (int v0 = wave; v0 < NT; v0 += 2 * nwaves) {
+        const int vv[2] = {v0, v0 + nwaves};
+        double x[2] = {0., 0.};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = vv[h];
+            if (!fail && v < NT) {
+                const unsigned* lo = s_h + (size_t)(2 * v) * nblk;
+                const unsigned* hi = lo + nblk;
+                for (int j = lane; j < nblk; j += 64) {
+                    const double y = __longlong_as_double((long long)(((unsigned long long)hi[j] << 32) | lo[j]));
+                    x[h] = (v < NV) ? x[h] + y : fmax(x[h], y);
+                }
             }
         }
-        x = (v < NV) ? wave_red_dpp<false>(x) : wave_red_dpp<true>(x);
-        if (lane == 63) s_tot[v] = x;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = vv[h];
+            if (v >= NT) break;                       // wave-uniform
+            x[h] = (v < NV) ? wave_red_dpp<false>(x[h]) : wave_red_dpp<true>(x[h]);
+            if (lane == 63) s_tot[v] = x[h];
+        }
     }
     __syncthreads();
     gen += 1u;
@@ -1047,8 +1054,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         for (int q = 0; q < 4; ++q) sq[q] += s_errb[q * B * CPT + c];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        double w = wave_sum(sq[q]);
-                        if (lane == 0) s_errs[b * 4 + q] = w;
+                        const double w = wave_red_dpp<false>(sq[q]);
+                        if (lane == 63) s_errs[b * 4 + q] = w;
                     }
                 }
                 __syncthreads();
