@@ -38,6 +38,8 @@ extern "C" {
 /* ---- geometry modules (reference: PHY_MODULE choice, global.h:5-19) ---- */
 #define XF_MA   0          /* bulb multi-angle, src/Phy_MA.cpp                  */
 #define XF_MAA  1          /* multi-azimuthal-angle, src/Phy_MAA.cpp            */
+#define XF_SA   2          /* single-angle bulb, src/Phy_SA.cpp: one trajectory,
+                              H_vv = coef * D(r) * X, dl = dr (Phy_SA.cpp:144-147,250-283) */
 
 /* ---- precision of the fused stepper ------------------------------------ */
 #define XF_FP64   0        /* FP64 state and arithmetic (parity mode)           */
@@ -52,7 +54,8 @@ extern "C" {
  * zero-initialised Util globals (src/Util.cpp:10-78).
  */
 typedef struct xf_params {
-    int    module;                 /* XF_MA or XF_MAA                          */
+    int    module;                 /* XF_MA, XF_MAA or XF_SA (SA: one beam,
+                                      Abins= and Pbins= not used for the state) */
     int    abins, pbins, ebins;    /* Abins= Pbins= Ebins=                     */
     int    flvs;                   /* Flvs= (only 2 is supported, as upstream) */
     int    spoints;                /* SPoints= (parsed, unused by MA/MAA)      */
@@ -151,7 +154,7 @@ int  xf_destroy(xf_ctx* ctx);                    /* phy::freemem, Phy_MA.cpp:290
 
 /* Dimension queries, Phy.h:24-67. */
 int  xf_beam_len(const xf_ctx* ctx);             /* phy::beamLen               */
-int  xf_get_dim(const xf_ctx* ctx);              /* phy::getDim (5 MA, 6 MAA)   */
+int  xf_get_dim(const xf_ctx* ctx);              /* phy::getDim (5 MA, 6 MAA, 4 SA) */
 int  xf_count_dim(const xf_ctx* ctx, size_t* count);   /* phy::countDim        */
 
 /* Host-side set-up values (for checks and output headers). */

@@ -91,6 +91,8 @@ def load(isa: str = "v3"):
     L.oracle_run.argtypes = [P, ctypes.c_int64, ctypes.c_void_p, dp]
     L.oracle_max_threads.restype = I
     L.oracle_set_threads.argtypes = [I]
+    L.oracle_sa_D.argtypes = [D, D]
+    L.oracle_sa_D.restype = D
     _LIB = L
     return L
 
@@ -128,7 +130,7 @@ class Oracle:
     def tables(self):
         fv = np.zeros((4, self.ne)); hv = np.zeros(2 * self.ne)
         le = np.zeros(4); ev = np.zeros(4)
-        c2 = np.zeros(self.p.abins); coef = np.zeros(1)
+        c2 = np.zeros(max(1, self.nbeam if self.p.module == 2 else self.p.abins)); coef = np.zeros(1)
         self.L.oracle_get_tables(self.h, _dp(fv), _dp(hv), _dp(le), _dp(ev), _dp(c2), _dp(coef))
         return dict(fv=fv, hvac=hv, L_E=le, Ev=ev, cos2R=c2, coef=coef[0])
 

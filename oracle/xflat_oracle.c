@@ -9,6 +9,7 @@
  *   src/NBeam.cpp   (spectra, H_vac, U, density, evolveBins*, calcESum, calcErrHvv)
  *   src/Phy_MA.cpp  (cos^2 grid, angle bins, delta-l, 2-moment H_vv)
  *   src/Phy_MAA.cpp (phi grid, trapezoidal delta-l, 4-moment H_vv)
+ *   src/Phy_SA.cpp  (one trajectory: H_vv = integ_coef * D(r) * source, dl = dr)
  *   src/Nmr.cpp     (the adaptive step sequence S1..S11, error, accept, dr)
  *   src/Matt.cpp, src/Fenergy.cpp, include/Util.h, include/global.h
  * Every function names the reference lines it follows.
@@ -99,11 +100,12 @@ double oracle_get_hm(const oracle_t* o, double radius) {
  *      init_cos2R Phy_MA.cpp:141-164, phy_maa::init/init_sincos2_R
  *      Phy_MAA.cpp:144-186,247-304, matt::init Matt.cpp:22-29 ------------- */
 oracle_t* oracle_create(const xf_params* p) {
-    if (p->flvs != 2 || p->ebins <= 0 || p->abins <= 0) return NULL;
+    if (p->flvs != 2 || p->ebins <= 0 || (p->module != XF_SA && p->abins <= 0)) return NULL;
     oracle_t* o = (oracle_t*)calloc(1, sizeof(oracle_t));
     o->p = *p;
     o->module = p->module;
-    o->nth = p->abins;
+    /* SA: one trajectory, beam_len = nbm_prtcls (Phy_SA.cpp:177) */
+    o->nth = (p->module == XF_SA) ? 1 : p->abins;
     o->nph = (p->module == XF_MAA) ? p->pbins : 1;
     if (o->nph <= 0) { free(o); return NULL; }
     o->ne = p->ebins;
@@ -155,6 +157,7 @@ oracle_t* oracle_create(const xf_params* p) {
         o->hvv_coeff = (O_SQRT2 * O_GF) / (4. * O_PI * O_PI * p->Rv * p->Rv);
         o->integ_coef = o->hvv_coeff * o->dPhi;
     } else {
+        /* Phy_MA.cpp:271-272; SA the same (Phy_SA.cpp:166-168) */
         o->hvv_coeff = (O_SQRT2 * O_GF) / (2. * O_PI * p->Rv * p->Rv);
         o->integ_coef = o->hvv_coeff;
     }
@@ -427,10 +430,14 @@ static void o_add_avg_nb(oracle_t* o, int in, int out, int nb) {
     }
 }
 
-/* ---- geometry: Phy_MA.cpp:172-196 / Phy_MAA.cpp:194-210 (angle bins) ---- */
+/* ---- geometry: Phy_MA.cpp:172-196 / Phy_MAA.cpp:194-210 (angle bins);
+ *      SA keeps the radius of the point (rarr, Phy_SA.cpp:119-122), here in
+ *      cos_t[pt] ----------------------------------------------------------- */
 void oracle_calc_angle_bins(oracle_t* o, double r, int pt) {
     const int nth = o->nth;
-    if (o->module == XF_MAA) {
+    if (o->module == XF_SA) {
+        o->cos_t[pt] = r;
+    } else if (o->module == XF_MAA) {
         double ra = o->p.Rv / r;
         double r2 = ra * ra;
 #pragma omp parallel for
@@ -452,12 +459,15 @@ void oracle_calc_angle_bins(oracle_t* o, double r, int pt) {
     }
 }
 
-/* Phy_MA.cpp:218-227 (dl = dr / cos at the start point) and
- * Phy_MAA.cpp:232-241 (trapezoid: dr / ((cos_s + cos_e) * .5)) */
+/* Phy_MA.cpp:218-227 (dl = dr / cos at the start point),
+ * Phy_MAA.cpp:232-241 (trapezoid: dr / ((cos_s + cos_e) * .5)) and
+ * Phy_SA.cpp:144-147 (dl = dr) */
 void oracle_calc_delta_ls(oracle_t* o, double dr, int cur, int s, int e) {
     const int nth = o->nth;
     for (int a = 0; a < nth; ++a) {
-        if (o->module == XF_MAA)
+        if (o->module == XF_SA)
+            o->dl[cur*nth + a] = dr;
+        else if (o->module == XF_MAA)
             o->dl[cur*nth + a] = dr / ( (o->cos_t[s*nth + a] + o->cos_t[e*nth + a])*.5 );
         else
             o->dl[cur*nth + a] = dr / ( (o->cos_t[s*nth + a] ) );
@@ -561,8 +571,28 @@ static void o_calc_hvv_maa(oracle_t* o, int set, int pt, double* hvv) {
         }
 }
 
+/* phy_sa::D, Phy_SA.cpp:250-256 */
+double oracle_sa_D(double Rv, double r) {
+    double R = Rv / r;
+    double tm = 1. - sqrt( 1. - (R*R) );
+    return 0.5 * (tm*tm);
+}
+
+/* phy_sa::calc_Hvv, Phy_SA.cpp:259-283: the four NBeams' energy sums combined
+ * by upd_nu_coef, times integ_coef * D(r) of the step point */
+static void o_calc_hvv_sa(oracle_t* o, int set, int pt, double* hvv) {
+    for (int z = 0; z < NSZ; ++z) hvv[z] = 0.;
+    for (int n = 0; n < 2; ++n) {
+        int neu = n << 1, aneu = neu + 1;
+        o_upd_nu_coef(o->hsum[set] + neu*4, o->hsum[set] + aneu*4, o->L_E[neu], o->L_E[aneu], hvv);
+    }
+    double coeff = o->integ_coef * oracle_sa_D(o->p.Rv, o->cos_t[pt]);
+    for (int z = 0; z < NSZ; ++z) hvv[z] *= coeff;
+}
+
 void oracle_calc_hvv(oracle_t* o, int set, int pt, int k) {
     if (o->module == XF_MAA) o_calc_hvv_maa(o, set, pt, o->hvv[k]);
+    else if (o->module == XF_SA) o_calc_hvv_sa(o, set, pt, o->hvv[k]);
     else o_calc_hvv_ma(o, set, pt, o->hvv[k]);
 }
 

@@ -4,9 +4,9 @@ Runs the reference's evolution path, compiled from its unmodified sources by
 oracle/Makefile into oracle/_ref/ (with the project's raw-file IO plugin
 oracle/ref_io_recorder.cpp), on small configurations:
 
-* MA cases start from a seeded perturbed state injected through the
+* MA and SA cases start from a seeded perturbed state injected through the
   reference's resume path (fio::fillInitData, src/IO_f.cpp:197-250, which sets
-  R0, dr and psi);
+  R0, dr and psi; SA calls it from Phy_SA.cpp:231);
 * MAA cases start from pure flavour states (the reference's MAA resume and
   perturbation are switched off, Phy_MAA.cpp:281,412).
 
@@ -48,18 +48,24 @@ CASES = {
     # MAA: 4-moment kernel, pure flavour initial states
     "maa_r50": (dict(Abins=8, Pbins=6, Ebins=12, R0=50, Ts=200), None, 0.0, 6),
     "maa_r10": (dict(Abins=8, Pbins=4, Ebins=12, R0=10, Ts=200, dm2="2.4E-3"), None, 0.0, 6),
+    # SA: one trajectory (reference built with -DSA), H_vv scaled by D(r)
+    "sa_r10_ih": (dict(_module="sa", Abins=1, Ebins=64, R0=10, Ts=400, dm2="-2.4E-3"), 21, 1e-6, 8),
+    "sa_r50_nh": (dict(_module="sa", Abins=1, Ebins=100, R0=50, Ts=300, dm2="2.4E-3", eps0="1.E-9"), 22, 1e-6, 8),
+    "sa_vac": (dict(_module="sa", Abins=1, Ebins=48, R0=30, Ts=200, hasMatter=0), 23, 1e-6, 6),
 }
 
 
 def _run_case(name, over, seed, amp, keep, isa="v3"):
     over = dict(over)
+    module = over.pop("_module", None)
     maa = int(over.get("Pbins", 1)) > 1
-    exe = os.path.join(REF_DIR, f"xflat_ref_{'maa' if maa else 'ma'}_{isa}")
+    sa = module == "sa"
+    exe = os.path.join(REF_DIR, f"xflat_ref_{'sa' if sa else 'maa' if maa else 'ma'}_{isa}")
     if not os.path.exists(exe):
         raise SystemExit(f"{exe} missing: run `make -C oracle ref` first")
     txt = config_text(**over)
     nth = int(over["Abins"]); nph = int(over.get("Pbins", 1)); ne = int(over["Ebins"])
-    nbeam = nth * nph
+    nbeam = 1 if sa else nth * nph
     r0 = float(over["R0"]); dr0 = 0.001
     if seed is not None:
         psi0 = seeded_psi(nbeam, ne, seed, amp)
@@ -105,7 +111,7 @@ def _run_case(name, over, seed, amp, keep, isa="v3"):
     nrec = len(its)
     sel = sorted(set(np.linspace(0, nrec - 1, keep).round().astype(int).tolist())) if nrec else []
     return dict(
-        config=np.array(txt), module=np.int64(1 if maa else 0),
+        config=np.array(txt), module=np.int64(2 if sa else 1 if maa else 0),
         psi0=psi0, r0=np.float64(r0), dr0=np.float64(dr0),
         it=np.array(its, dtype=np.int64), r=np.array(rs), dr=np.array(drs),
         psi_idx=np.array(sel, dtype=np.int64),

@@ -97,3 +97,32 @@ def test_host_snapshots_probability_and_resume_match_oracle(tmp_path):
     o3.run(int(k_last) + 1)
     opsi3, _, _, _ = o3.state()
     assert np.abs(_pee(last2) - _pee(opsi3)).max() <= 1e-9
+
+
+def test_host_single_angle_module_matches_oracle(tmp_path):
+    """--module sa (the reference's -DSA build, Phy_SA.cpp): one trajectory,
+    a Snapshot without the theta dimension, every accepted step recorded."""
+    from xflat_amd.config import config_text, parse_config_text, XF_SA
+    from oracle.oracle import Oracle
+    from xflat_amd.seed import pure_psi
+
+    over = dict(Abins=1, Ebins=40, R0=30, Rn=250, dr="1e-3", eps0="1.E-8", dumpMode=1,
+                filePrefix=str(tmp_path) + "/", itr_step1=1, r_step1="1e9", t_step1=100000,
+                newFile_step=1000000, sync_step=7)
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(config_text(Ts=120, **over))
+    out = _run([str(cfg), "--module", "sa", "--chunk", "29"], str(tmp_path))
+    assert "Total iterations: 120" in out
+    p = parse_config_text(config_text(Ts=120, **over), module=XF_SA)
+    o = Oracle(p)
+    o.set_state(pure_psi(1, 40), 30.0, 1e-3)
+    n, rec, _ = o.run(120)
+    acc = np.nonzero(rec["accepted"])[0]
+    with netcdf_file(str(tmp_path / "Snapshot0_0_.nc"), "r", mmap=False) as nc:
+        assert list(nc.dimensions) == ["r", "prtcl", "comp", "ebin"]
+        r = nc.variables["r"][:].copy()
+        snaps = nc.variables["psi"][:].copy()
+    assert len(r) == len(acc)
+    np.testing.assert_allclose(r, rec["r"][acc] + rec["dr"][acc], rtol=1e-12)
+    opsi, orr, _, _ = o.state()
+    assert np.abs(_pee(snaps[-1][None]) - _pee(opsi)).max() <= 1e-9

@@ -147,11 +147,14 @@ def test_fused_modes_agree(prec):
             assert dpee <= 1e-5, key
 
 
-def test_phy_mirror_one_step_matches_oracle():
-    """The Phy.h mirror, driven call by call like Nmr.cpp:153-206."""
+@pytest.mark.parametrize("name", ["ma_r50_ih.npz", "sa_r50_nh.npz"])
+def test_phy_mirror_one_step_matches_oracle(name):
+    """The Phy.h mirror, driven call by call like Nmr.cpp:153-206 (MA, and
+    the single-angle module Phy_SA.cpp)."""
     capi = _capi()
     from oracle.oracle import Oracle
-    g, p = _golden("ma_r50_ih.npz")
+    g, p = _golden(name)
+    print(f"[phy mirror] {name}", flush=True)
     o = Oracle(p)
     psi0 = g["psi0"]
     r, dr = 50.0, 2e-4
@@ -210,10 +213,12 @@ def test_fused_vs_oracle_bulb_200x100():
     assert np.abs(pe - oe).max() <= PEE_TOL and np.abs(pb - ob).max() <= PEE_TOL
 
 
+@pytest.mark.parametrize("name", ["ma_r50_ih.npz", "sa_r10_ih.npz"])
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
-def test_mixed_precision_close_to_fp64(prec):
+def test_mixed_precision_close_to_fp64(prec, name):
     capi = _capi()
-    g, p = _golden("ma_r50_ih.npz")
+    g, p = _golden(name)
+    print(f"[{prec}] {name}", flush=True)
     pr = capi.XF_MIXED if prec == "mixed" else capi.XF_FP32
     with capi.Context(p, precision=capi.XF_FP64) as a, capi.Context(p, precision=pr) as b:
         for c in (a, b):

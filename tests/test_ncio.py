@@ -11,16 +11,15 @@ import pytest
 from scipy.io import netcdf_file
 
 from xflat_amd import capi
-from xflat_amd.config import config_text, parse_config_text
+from xflat_amd.config import config_text, parse_config_text, n_beams, XF_SA
 
 
-def _params(**over):
-    return parse_config_text(config_text(**over))
+def _params(module=None, **over):
+    return parse_config_text(config_text(**over), module=module)
 
 
 def _rand_psi(p, rng):
-    nb = p.abins * (p.pbins if p.module == 1 else 1)
-    return rng.standard_normal((nb, 4, 4, p.ebins))
+    return rng.standard_normal((n_beams(p), 4, 4, p.ebins))
 
 
 def test_snapshot_schema_and_records_ma(tmp_path):
@@ -69,6 +68,35 @@ def test_snapshot_schema_maa(tmp_path):
         v = nc.variables["psi"][:]
         assert v.shape == (1, 3, 4, 4, 4, 2)
         np.testing.assert_array_equal(v[0].reshape(12, 4, 4, 2), x)
+
+
+def test_snapshot_probability_and_resume_sa(tmp_path):
+    """SA (Phy_SA.cpp): one trajectory; the Snapshot has no theta dimension
+    (getDimInfo, Phy_SA.cpp:74-80) and the Probability grid is 1 x 1
+    (theta = beamLen / (Pbins * 4), IO_f.cpp:93-94)."""
+    p = _params(module=XF_SA, Abins=7, Ebins=5)
+    assert p.module == XF_SA and n_beams(p) == 1
+    rng = np.random.default_rng(9)
+    xs = [_rand_psi(p, rng) for _ in range(2)]
+    fv = rng.random((4, p.ebins))
+    snap, prob = str(tmp_path / "snap.nc"), str(tmp_path / "prob.nc")
+    with capi.NcFile(p, snap) as f, capi.NcFile(p, prob, probability=True) as g:
+        for k, x in enumerate(xs):
+            f.append_snapshot(10.0 + k, 1e-6 * (k + 1), x)
+            g.append_probability(10.0 + k, x, fv)
+    with netcdf_file(snap, "r", mmap=False) as nc:
+        assert list(nc.dimensions) == ["r", "prtcl", "comp", "ebin"]
+        v = nc.variables["psi"]
+        assert v.dimensions == ("r", "prtcl", "comp", "ebin")
+        np.testing.assert_array_equal(v[1], xs[1][0])
+    with netcdf_file(prob, "r", mmap=False) as nc:
+        assert list(nc.dimensions) == ["r", "theta", "phi"]
+        assert (nc.dimensions["theta"], nc.dimensions["phi"]) == (1, 1)
+        want = ((xs[0][0, 0, 0] ** 2 + xs[0][0, 0, 1] ** 2) * fv[0]).sum()
+        np.testing.assert_allclose(nc.variables["prtcl0"][0, 0, 0], want, rtol=1e-14)
+    psi, r, dr = capi.snapshot_read_last(snap, p)
+    assert (r, dr) == (11.0, 2e-6)
+    np.testing.assert_array_equal(psi, xs[1])
 
 
 def test_probability_values(tmp_path):

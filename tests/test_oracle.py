@@ -8,7 +8,9 @@
    U = expm(-i H dl) (NBeam.cpp:360-393), the angular moment factorisation vs
    the brute-force (1 - v.v') double sum (Phy_MA.cpp:372-446,
    Phy_MAA.cpp:445-562), the SU(2)-partner invariance of the nu_x columns,
-   and the two-flavour vacuum oscillation along the bulb's straight rays.
+   the two-flavour vacuum oscillation along the bulb's straight rays, and the
+   single-angle H_vv (Phy_SA.cpp:250-283) as the continuum limit of the
+   multi-angle integral on the radial trajectory.
 """
 import os
 
@@ -18,7 +20,7 @@ import scipy.linalg
 
 from conftest import GOLDEN, golden_files
 from oracle import oracle as O
-from xflat_amd.config import parse_config_text, bulb_params, XF_MA, XF_MAA
+from xflat_amd.config import parse_config_text, bulb_params, XF_MA, XF_MAA, XF_SA
 from xflat_amd.seed import seeded_psi
 
 
@@ -182,3 +184,47 @@ def test_vacuum_oscillation_closed_form():
     got = psi[:, 0, 0] ** 2 + psi[:, 0, 1] ** 2
     assert r > 21.0
     np.testing.assert_allclose(got, pee, atol=2e-6)
+
+
+def _radial_beam_hvv(state, nth, r):
+    ne = state.shape[-1]
+    pm = bulb_params(module=XF_MA, Abins=nth, Ebins=ne, R0=10)
+    om = O.Oracle(pm)
+    om.set_psi(0, np.broadcast_to(state, (nth, 4, 4, ne)).copy())
+    om.calc_esum(0)
+    om.calc_angle_bins(r, 1)
+    om.calc_hvv(0, 1, 2)
+    return om.get_hvv(2)[0]
+
+
+@pytest.mark.parametrize("r", [10.0, 14.0, 40.0])
+def test_sa_hvv_is_radial_limit_of_multi_angle(r):
+    """With the same source X on every angle bin, the MA H_vv of the beam
+    nearest the radial direction is coef * sum_j w_j X (1 - c_j c_0), whose
+    continuum limit (c_0 -> 1, sum -> integral over c from sqrt(1 - (Rv/r)^2)
+    to 1 of (1 - c) dc) is coef * D(r) * X with D = (1 - sqrt(1 - (Rv/r)^2))^2 / 2:
+    the single-angle H_vv.  The MA value must converge to the SA one as the
+    angle grid is refined -- first order in 1/Abins for r > Rv, half order at
+    r = Rv where the weights 1/c_j are singular at the grid's edge -- which
+    checks D(r), the SA coefficient and the sign conventions against an
+    independent (multi-angle) code path."""
+    ne = 6
+    rng = np.random.default_rng(17)
+    state = rng.normal(size=(4, 4, ne))               # one beam's 4 NBeams
+    pa = bulb_params(module=XF_SA, Abins=1, Ebins=ne, R0=10)
+    oa = O.Oracle(pa)
+    assert oa.nbeam == 1
+    oa.set_psi(0, state[None])
+    oa.calc_esum(0)
+    oa.calc_angle_bins(r, 1)
+    oa.calc_hvv(0, 1, 2)
+    sa = oa.get_hvv(2)[0][[0, 2, 3]]
+    D = oa.L.oracle_sa_D(pa.Rv, r)
+    assert abs(D - 0.5 * (1 - np.sqrt(1 - (pa.Rv / r) ** 2)) ** 2) < 1e-15
+    scale = np.abs(sa).max()
+    assert scale > 0
+    err = [np.abs(_radial_beam_hvv(state, n, r)[[0, 2, 3]] - sa).max() / scale for n in (5000, 20000)]
+    if r > pa.Rv:
+        assert err[1] < 1e-4 and err[1] < 0.3 * err[0], err
+    else:
+        assert err[1] < 5e-3 and err[1] < 0.6 * err[0], err

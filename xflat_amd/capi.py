@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .config import XfParams, XF_FP64, XF_MIXED, XF_FP32  # noqa: F401
+from .config import XfParams, XF_FP64, XF_MIXED, XF_FP32, n_beams  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libxflat.so")
@@ -182,7 +182,7 @@ class Context:
         h = ctypes.c_void_p()
         check(self.L.xf_create(ctypes.byref(params), ctypes.byref(self.opts), ctypes.byref(h)))
         self.h = h
-        self.nbeam = params.abins * (params.pbins if params.module == 1 else 1)
+        self.nbeam = n_beams(params)
         self.ne = params.ebins
         self.shape = (self.nbeam, 4, 4, self.ne)
 
@@ -331,7 +331,7 @@ class Ensemble:
 
     def shape(self, i):
         p = self.params[i]
-        nb = p.abins * (p.pbins if p.module == 1 else 1)
+        nb = n_beams(p)
         return (nb, 4, 4, p.ebins)
 
     def upload(self, i, psi, r, dr):
@@ -410,7 +410,7 @@ class NcFile:
 
 def snapshot_read_last(path: str, p: XfParams):
     """Last record (psi, r, dr) of a Snapshot file (the resume read)."""
-    nb = p.abins * (p.pbins if p.module == 1 else 1)
+    nb = n_beams(p)
     psi = np.zeros((nb, 4, 4, p.ebins))
     r = ctypes.c_double()
     dr = ctypes.c_double()

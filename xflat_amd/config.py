@@ -14,7 +14,7 @@ MEV = 5.067731161976211e+15     # MeV -> 1/km
 MEV_2 = 3.893793036626876e-32   # MeV^-2 -> km^2
 EV2 = 2.568189913006477e+19     # eV^2 -> 1/km^2
 
-XF_MA, XF_MAA = 0, 1
+XF_MA, XF_MAA, XF_SA = 0, 1, 2
 XF_FP64, XF_MIXED, XF_FP32 = 0, 1, 2
 
 
@@ -130,10 +130,18 @@ def _c_double(tok: str) -> float:
     raise ValueError(tok)
 
 
+def n_beams(p) -> int:
+    """Beams of the state: Abins x Pbins for MAA, Abins for MA and one
+    trajectory for SA (beam_len = nbm_prtcls, src/Phy_SA.cpp:177)."""
+    if p.module == XF_SA:
+        return 1
+    return p.abins * (p.pbins if p.module == XF_MAA else 1)
+
+
 def parse_config_text(text: str, module: int | None = None) -> XfParams:
     """Parse config text like Parser::readLines (src/Parser.cpp:25-127).
 
-    ``module`` selects MA/MAA (a compile-time choice in the reference,
+    ``module`` selects MA/MAA/SA (a compile-time choice in the reference,
     global.h:5-19); by default it is MAA when Pbins > 1, else MA.
     """
     p = XfParams()
@@ -201,6 +209,6 @@ def config_text(**overrides) -> str:
 def bulb_params(**overrides) -> XfParams:
     """Parsed XfParams for the bulb physics with keyword overrides.
 
-    MAA is selected when Pbins > 1 (or pass module=XF_MA/XF_MAA)."""
+    MAA is selected when Pbins > 1 (or pass module=XF_MA/XF_MAA/XF_SA)."""
     module = overrides.pop("module", None)
     return parse_config_text(config_text(**overrides), module=module)

@@ -626,13 +626,17 @@ int xf_destroy(xf_ctx* ctx) {
 }
 
 int xf_beam_len(const xf_ctx* ctx) { return ctx ? ctx->t.nbeam * 4 : 0; }
-int xf_get_dim(const xf_ctx* ctx) { return ctx ? (ctx->t.module == XF_MAA ? 6 : 5) : 0; }
+int xf_get_dim(const xf_ctx* ctx) {
+    if (!ctx) return 0;
+    return ctx->t.module == XF_MAA ? 6 : ctx->t.module == XF_SA ? 4 : 5;
+}
 int xf_count_dim(const xf_ctx* ctx, size_t* count) {
     if (!ctx || !count) return xf_fail(XF_EINVAL, "xf_count_dim: null argument");
-    // [r, theta, (phi,) prtcl, comp, ebin] (Phy_MA.cpp:253-258, Phy_MAA.cpp:269-275)
+    // [r, theta, (phi,) prtcl, comp, ebin] (Phy_MA.cpp:253-258, Phy_MAA.cpp:269-275);
+    // SA: [r, prtcl, comp, ebin] (Phy_SA.cpp:74-80)
     int k = 0;
     count[k++] = 1;
-    count[k++] = ctx->t.nth;
+    if (ctx->t.module != XF_SA) count[k++] = ctx->t.nth;
     if (ctx->t.module == XF_MAA) count[k++] = ctx->t.nph;
     count[k++] = 4;
     count[k++] = 4;

@@ -162,8 +162,13 @@ int put_record(xf_nc* f, const std::vector<const double*>& src) {
 }
 
 bool check_params(const xf_params* p) {
-    return p && p->abins > 0 && p->ebins > 0 && (p->module == XF_MA || p->pbins > 0);
+    if (!p || p->ebins <= 0) return false;
+    if (p->module == XF_SA) return true;
+    return p->abins > 0 && (p->module == XF_MA || (p->module == XF_MAA && p->pbins > 0));
 }
+// beams of the state: theta x phi (MA, MAA); one trajectory for SA
+int n_theta(const xf_params* p) { return p->module == XF_SA ? 1 : p->abins; }
+int n_phi(const xf_params* p) { return p->module == XF_MAA ? p->pbins : 1; }
 
 }  // namespace
 
@@ -181,13 +186,13 @@ int xf_snapshot_create(const xf_params* p, const char* path, xf_nc** out) {
     std::unique_ptr<xf_nc> f(new xf_nc());
     const bool maa = p->module == XF_MAA;
     f->kind = 0;
-    f->nth = p->abins;
-    f->nph = maa ? p->pbins : 1;
+    f->nth = n_theta(p);
+    f->nph = n_phi(p);
     f->ne = p->ebins;
     f->nbeam = f->nth * f->nph;
-    // getDimInfo: Phy_MA.cpp:92-99, Phy_MAA.cpp:94-102
+    // getDimInfo: Phy_MA.cpp:92-99, Phy_MAA.cpp:94-102, Phy_SA.cpp:74-80 (no theta)
     f->dims.push_back({"r", 0});
-    f->dims.push_back({"theta", (uint64_t)f->nth});
+    if (p->module != XF_SA) f->dims.push_back({"theta", (uint64_t)f->nth});
     if (maa) f->dims.push_back({"phi", (uint64_t)f->nph});
     f->dims.push_back({"prtcl", (uint64_t)xfc::NPR});
     f->dims.push_back({"comp", 4});
@@ -225,8 +230,10 @@ int xf_probability_create(const xf_params* p, const char* path, xf_nc** out) {
     if (!check_params(p) || !path || !out) return xf_fail(XF_EINVAL, "xf_probability_create: bad arguments");
     std::unique_ptr<xf_nc> f(new xf_nc());
     f->kind = 1;
-    f->nth = p->abins;
-    f->nph = p->module == XF_MAA ? p->pbins : 1;
+    // theta = beamLen / (Pbins * 4), phi = Pbins (IO_f.cpp:93-94,164-166): the
+    // beams of the state; for SA one trajectory
+    f->nth = n_theta(p);
+    f->nph = n_phi(p);
     f->ne = p->ebins;
     f->nbeam = f->nth * f->nph;
     f->dims = {{"r", 0}, {"theta", (uint64_t)f->nth}, {"phi", (uint64_t)f->nph}};
@@ -390,8 +397,7 @@ int xf_snapshot_read_last(const char* path, const xf_params* p, double* psi, dou
         numrecs = (sz > (long)first && recsize) ? (uint32_t)(((uint64_t)sz - first) / recsize) : 0;
         if (numrecs == 0) return bad("no records");
     }
-    const int nph = p->module == XF_MAA ? p->pbins : 1;
-    const uint64_t want = (uint64_t)p->abins * nph * xfc::NPR * 4 * p->ebins;
+    const uint64_t want = (uint64_t)n_theta(p) * n_phi(p) * xfc::NPR * 4 * p->ebins;
     uint64_t have = 1;
     for (size_t k = 1; k < vars[ip].dims.size(); ++k) have *= dims[vars[ip].dims[k]].len;
     if (have != want) return bad("psi shape does not match the configuration");

@@ -172,12 +172,15 @@ __global__ void __launch_bounds__(256) k_nbeam(XfPhyDev P, const double* __restr
     }
 }
 
-// phy::calcAngleBins (Phy_MA.cpp:172-196, Phy_MAA.cpp:194-210)
+// phy::calcAngleBins (Phy_MA.cpp:172-196, Phy_MAA.cpp:194-210); SA keeps the
+// radius of the point instead (rarr, Phy_SA.cpp:119-122), here in cos_t.
 __global__ void k_angle_bins(XfPhyDev P, double r, int pt) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.nth) return;
     int idx = pt * P.nth + a;
-    if (P.module == XF_MAA) {
+    if (P.module == XF_SA) {
+        P.cos_t[idx] = r;
+    } else if (P.module == XF_MAA) {
         double ra = P.Rv / r;
         double r2 = ra * ra;
         P.cos_t[idx] = sqrt(1. - r2 * (1. - P.x[a]));
@@ -191,11 +194,14 @@ __global__ void k_angle_bins(XfPhyDev P, double r, int pt) {
     }
 }
 
-// phy::calcDeltaLs (Phy_MA.cpp:218-227, Phy_MAA.cpp:232-241)
+// phy::calcDeltaLs (Phy_MA.cpp:218-227, Phy_MAA.cpp:232-241; SA: dl = dr,
+// Phy_SA.cpp:144-147)
 __global__ void k_delta_ls(XfPhyDev P, double dr, int cur, int s, int e) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.nth) return;
-    if (P.module == XF_MAA)
+    if (P.module == XF_SA)
+        P.dl[cur * P.nth + a] = dr;
+    else if (P.module == XF_MAA)
         P.dl[cur * P.nth + a] = dr / ((P.cos_t[s * P.nth + a] + P.cos_t[e * P.nth + a]) * .5);
     else
         P.dl[cur * P.nth + a] = dr / ((P.cos_t[s * P.nth + a]));
@@ -246,6 +252,11 @@ __global__ void k_hvv_partial(XfPhyDev P, const double* __restrict__ hsum, int p
             const double* ra = hsum + ((size_t)a * 4 + aneu) * 4;
             upd(rn, ra, P.L_E[neu], P.L_E[aneu], np);
         }
+        if (P.module == XF_SA) {
+            // SA: the source itself (Phy_SA.cpp:259-278)
+            for (int z = 0; z < 4; ++z) part[a * 16 + z] = np[z];
+            return;
+        }
         for (int z = 0; z < 4; ++z) {
             np[z] *= P.dcos_t[aid];
             part[a * 16 + z] = np[z];
@@ -258,7 +269,7 @@ __global__ void k_hvv_partial(XfPhyDev P, const double* __restrict__ hsum, int p
 // getHvv for every beam (Phy_MA.cpp:372-377,470-473; Phy_MAA.cpp:445-461,611-619).
 __global__ void k_hvv_final(XfPhyDev P, const double* __restrict__ part, int pt, double* __restrict__ hvv) {
     __shared__ double J[16];
-    const int nv = (P.module == XF_MAA) ? 16 : 8;
+    const int nv = (P.module == XF_MAA) ? 16 : (P.module == XF_SA) ? 4 : 8;
     if ((int)threadIdx.x < nv) {
         double s = 0.;
         for (int a = 0; a < P.nth; ++a) s += part[a * 16 + threadIdx.x];
@@ -269,7 +280,14 @@ __global__ void k_hvv_final(XfPhyDev P, const double* __restrict__ part, int pt,
         int th = b / P.nph;
         int aid = pt * P.nth + th;
         double* ret = hvv + b * 4;
-        if (P.module == XF_MAA) {
+        if (P.module == XF_SA) {
+            // hvv *= integ_coef * D(r), D(r) = (1 - sqrt(1 - (Rv/r)^2))^2 / 2
+            // (Phy_SA.cpp:250-256,280-281); cos_t holds the point's radius
+            const double R = P.Rv / P.cos_t[aid];
+            const double tm = 1. - sqrt(1. - (R * R));
+            const double coeff = P.coef * (.5 * (tm * tm));
+            for (int z = 0; z < 4; ++z) ret[z] = J[z] * coeff;
+        } else if (P.module == XF_MAA) {
             int q = b - th * P.nph;
             double st_ = P.sin_t[aid], ct = P.cos_t[aid], sp = P.sphi[q], cp = P.cphi[q];
             for (int z = 0; z < 4; ++z)

@@ -141,12 +141,14 @@ static double avg_energy(const xf_params* p, int s) {
 int xf_build_tables(const xf_params* p, XfTables* t) {
     using namespace xfc;
     if (p->flvs != 2) return xf_fail(XF_EINVAL, "Flvs= %d: only two flavours are supported", p->flvs);
-    if (p->module != XF_MA && p->module != XF_MAA) return xf_fail(XF_EINVAL, "unknown module %d", p->module);
-    if (p->abins <= 0 || p->ebins <= 0) return xf_fail(XF_EINVAL, "Abins=/Ebins= must be positive");
+    if (p->module != XF_MA && p->module != XF_MAA && p->module != XF_SA)
+        return xf_fail(XF_EINVAL, "unknown module %d", p->module);
+    if ((p->module != XF_SA && p->abins <= 0) || p->ebins <= 0) return xf_fail(XF_EINVAL, "Abins=/Ebins= must be positive");
     if (p->module == XF_MAA && p->pbins <= 0) return xf_fail(XF_EINVAL, "Pbins= must be positive for MAA");
     if (p->Rv <= 0) return xf_fail(XF_EINVAL, "Rv= must be positive");
     t->module = p->module;
-    t->nth = p->abins;
+    // SA evolves one trajectory: beam_len = nbm_prtcls (Phy_SA.cpp:177)
+    t->nth = (p->module == XF_SA) ? 1 : p->abins;
     t->nph = (p->module == XF_MAA) ? p->pbins : 1;
     t->ne = p->ebins;
     t->nbeam = t->nth * t->nph;
@@ -202,7 +204,9 @@ int xf_build_tables(const xf_params* p, XfTables* t) {
         double hvv_coeff = (SQRT2 * GF) / (4. * PI * PI * p->Rv * p->Rv);
         t->integ_coef = hvv_coeff * t->dPhi;
     } else {
-        t->integ_coef = (SQRT2 * GF) / (2. * PI * p->Rv * p->Rv);   // Phy_MA.cpp:271-272
+        // Phy_MA.cpp:271-272; SA: the same coefficient (Phy_SA.cpp:166-168),
+        // times D(r) per step point
+        t->integ_coef = (SQRT2 * GF) / (2. * PI * p->Rv * p->Rv);
     }
     for (int s = 0; s < NPR; ++s) t->L_E[s] = p->L[s] / t->Ev[s];   // Phy_MA.cpp:280-282
 

@@ -13,8 +13,18 @@ XfStepFn xf_pick_mixed_ma(int, int, int);
 XfStepFn xf_pick_mixed_maa(int, int, int);
 XfStepFn xf_pick_fp32_ma(int, int, int);
 XfStepFn xf_pick_fp32_maa(int, int, int);
+XfStepFn xf_pick_fp64_sa(int, int, int);
+XfStepFn xf_pick_mixed_sa(int, int, int);
+XfStepFn xf_pick_fp32_sa(int, int, int);
 
 static XfStepFn pick(const XfLaunch& L, int geom) {
+    if (geom == XF_SA) {
+        switch (L.precision) {
+            case XF_FP64: return xf_pick_fp64_sa(L.ns, L.cpt, L.mode);
+            case XF_FP32: return xf_pick_fp32_sa(L.ns, L.cpt, L.mode);
+            default: return xf_pick_mixed_sa(L.ns, L.cpt, L.mode);
+        }
+    }
     bool maa = geom == XF_MAA;
     switch (L.precision) {
         case XF_FP64: return maa ? xf_pick_fp64_maa(L.ns, L.cpt, L.mode) : xf_pick_fp64_ma(L.ns, L.cpt, L.mode);
@@ -26,7 +36,7 @@ static XfStepFn pick(const XfLaunch& L, int geom) {
 bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nullptr; }
 
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
-    int nm = geom == XF_MAA ? 4 : 2;
+    int nm = geom == XF_MAA ? 4 : geom == XF_SA ? 1 : 2;
     int bsize = 3 + 3 * nm + 9 + 12 + 4;
     // the pass-C error buffer doubles as the exchange staging: the lanes'
     // values (in chunks of values) and the fetched rows [2 * 3 * nm][xrows]

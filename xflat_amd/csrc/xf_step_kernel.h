@@ -252,7 +252,7 @@ __device__ __forceinline__ double hm_at(const XfInstDev& I, double r) {
 
 // Per-beam record in LDS (doubles).
 template <int GEOM> struct BRec {
-    static constexpr int NM = (GEOM == XF_MAA) ? 4 : 2;
+    static constexpr int NM = (GEOM == XF_MAA) ? 4 : (GEOM == XF_SA) ? 1 : 2;
     static constexpr int DL = 0;                  // dl[3]
     static constexpr int W = 3;                   // W[3 pts][NM] moment weights
     static constexpr int G = W + 3 * NM;          // G[3 pts][3] hvv geometry
@@ -263,10 +263,29 @@ template <int GEOM> struct BRec {
 
 // Geometry of one beam at r, r+dr/2, r+dr: angle bins (Phy_MA.cpp:172-196,
 // Phy_MAA.cpp:194-210) and path lengths (Phy_MA.cpp:218-227, Phy_MAA.cpp:232-241).
+// SA (one trajectory): the moment is the source itself (weight 1), the
+// geometry slot holds D(r) = (1 - sqrt(1 - (Rv/r)^2))^2 / 2 (Phy_SA.cpp:250-256)
+// and dl is the sub-step length itself (Phy_SA.cpp:144-147).
 template <int GEOM>
 __device__ void beam_geometry(const XfInstDev& I, int b, double r, double rdr2, double rdr, double dr,
                               double dr2, double* rec) {
     using R = BRec<GEOM>;
+    if constexpr (GEOM == XF_SA) {
+        const double rp[3] = {r, rdr2, rdr};
+#pragma unroll
+        for (int pt = 0; pt < 3; ++pt) {
+            const double Ra = I.Rv / rp[pt];
+            const double tm = 1. - sqrt(1. - (Ra * Ra));
+            rec[R::W + pt] = 1.;
+            rec[R::G + pt * 3 + 0] = .5 * (tm * tm);
+            rec[R::G + pt * 3 + 1] = 0.;
+            rec[R::G + pt * 3 + 2] = 0.;
+        }
+        rec[R::DL + 0] = dr2;
+        rec[R::DL + 1] = dr;
+        rec[R::DL + 2] = dr2;
+        return;
+    }
     int th = (GEOM == XF_MAA) ? b / I.nph : b;
     double x = I.x[th], dx = I.dx[th];
     double rp[3] = {r, rdr2, rdr};
@@ -323,8 +342,10 @@ __device__ __forceinline__ void beam_h(const double* rec, int pt, const double* 
     double hv[3];
 #pragma unroll
     for (int z = 0; z < 3; ++z) {
-        if (GEOM == XF_MAA)
+        if constexpr (GEOM == XF_MAA)
             hv[z] = (M[z] - (G[0] * M[3 + z] + G[1] * M[6 + z] + G[2] * M[9 + z])) * coef;
+        else if constexpr (GEOM == XF_SA)
+            hv[z] = M[z] * (coef * G[0]);             // hvv *= integ_coef * D(r), Phy_SA.cpp:280-281
         else
             hv[z] = (M[z] - (M[3 + z] * G[0])) * coef;
     }
@@ -491,9 +512,11 @@ __device__ __forceinline__ double lane63(double v) {
 // every row of gen + 1 arrived, and a workgroup publishes gen + 1 only after
 // its fetch of gen returned (the published values depend on the fetched
 // ones).  All waits are bounded.
+// Step 1 (xch_publish) and steps 2-4 (xch_gather), run back to back by
+// reduce_publish.
 template <int NV, int NMAX>
-__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot, double* s_x, int xcap, const XfInstDev& I,
-                                               int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr) {
+__device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int xcap, const XfInstDev& I, int lb,
+                                            unsigned gen, long long* tr = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int B = blockDim.x;
     const int nblk = I.nblk;
@@ -503,7 +526,6 @@ __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot,
     const unsigned long long tagw = (unsigned long long)tag << 32;
     const size_t rep_words = (size_t)XF_ROWW * nblk;
     unsigned long long* rows0 = I.rows + (size_t)(gen & 1u) * RP * rep_words;
-    unsigned long long* rows = rows0 + (size_t)(lb % RP) * rep_words;
     if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
     const int vg = max(1, min(NT, xcap / B));          // values staged at once
     for (int v0 = 0; v0 < NT; v0 += vg) {
@@ -542,6 +564,20 @@ __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot,
         }
     }
     if (tr && tid == 0) tr[1] = (long long)__builtin_amdgcn_s_memrealtime();
+}
+
+template <int NV, int NMAX>
+__device__ __forceinline__ bool xch_gather(double* s_tot, double* s_x, const XfInstDev& I, int lb, unsigned& gen,
+                                           int* status, int* s_flag, long long* tr = nullptr) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int B = blockDim.x;
+    const int nblk = I.nblk;
+    constexpr int NT = NV + NMAX;
+    constexpr int RP = XF_ROW_REPLICAS;
+    const unsigned tag = gen + 1u;
+    const unsigned long long tagw = (unsigned long long)tag << 32;
+    const size_t rep_words = (size_t)XF_ROWW * nblk;
+    unsigned long long* rows = I.rows + (size_t)(gen & 1u) * RP * rep_words + (size_t)(lb % RP) * rep_words;
     if (tid < 64) {
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
         int fail = 0;
@@ -611,6 +647,13 @@ __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot,
     gen += 1u;
     if (tr && tid == 0) tr[3] = (long long)__builtin_amdgcn_s_memrealtime();
     return !fail;
+}
+
+template <int NV, int NMAX>
+__device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot, double* s_x, int xcap, const XfInstDev& I,
+                                               int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr) {
+    xch_publish<NV, NMAX>(acc, s_x, xcap, I, lb, gen, tr);
+    return xch_gather<NV, NMAX>(s_tot, s_x, I, lb, gen, status, s_flag, tr);
 }
 }  // namespace
 

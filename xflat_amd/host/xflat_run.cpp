@@ -1,7 +1,10 @@
 // xflat_run -- command-line host for the MI355X evolution step.
 //
 //   xflat_run <config.txt> [snapshot.nc] [--precision fp64|mixed|fp32]
-//             [--chunk N] [--species 2|4] [--quiet]
+//             [--module ma|maa|sa] [--chunk N] [--species 2|4] [--quiet]
+//
+// --module stands for the reference's compile-time module choice (-DMA,
+// -DMAA, -DSA, include/global.h:5-19); by default MAA when Pbins > 1, else MA.
 //
 // The reference's driver (src/main.cpp:11-38: argv[1] = config file,
 // optional argv[2] = a Snapshot file to resume from) with its evolution loop
@@ -74,13 +77,20 @@ struct Stream {
 int main(int argc, char** argv) {
     const char* cfg = nullptr;
     const char* init = nullptr;
-    int precision = XF_FP64, species = 0, chunk = 2000;
+    int precision = XF_FP64, species = 0, chunk = 2000, module = -1;
     bool quiet = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--precision" && i + 1 < argc) {
             std::string v = argv[++i];
             precision = v == "mixed" ? XF_MIXED : v == "fp32" ? XF_FP32 : XF_FP64;
+        } else if (a == "--module" && i + 1 < argc) {
+            std::string v = argv[++i];
+            module = v == "sa" ? XF_SA : v == "maa" ? XF_MAA : v == "ma" ? XF_MA : -2;
+            if (module == -2) {
+                std::fprintf(stderr, "xflat_run: --module must be ma, maa or sa\n");
+                return 2;
+            }
         } else if (a == "--chunk" && i + 1 < argc) {
             chunk = std::max(1, std::atoi(argv[++i]));
         } else if (a == "--species" && i + 1 < argc) {
@@ -98,12 +108,13 @@ int main(int argc, char** argv) {
     }
     if (!cfg) {
         std::fprintf(stderr, "usage: xflat_run <config.txt> [snapshot.nc] [--precision fp64|mixed|fp32] "
-                             "[--chunk N] [--species 2|4] [--quiet]\n");
+                             "[--module ma|maa|sa] [--chunk N] [--species 2|4] [--quiet]\n");
         return 2;
     }
 
     xf_params p;
     CHECK(xf_parse_config(cfg, &p));
+    if (module >= 0) p.module = module;
     xf_opts o;
     std::memset(&o, 0, sizeof o);
     o.precision = precision;
@@ -113,8 +124,7 @@ int main(int argc, char** argv) {
     xf_ctx* ctx = nullptr;
     CHECK(xf_create(&p, &o, &ctx));
 
-    const int nph = p.module == XF_MAA ? p.pbins : 1;
-    const size_t ncell = (size_t)p.abins * nph * 4 * 4 * p.ebins;
+    const size_t ncell = (size_t)xf_beam_len(ctx) * 4 * p.ebins;    // beams x prtcl x comp x ebin
     std::vector<double> psi(ncell);
     double r = p.R0, dr = p.dr;
     if (init) {
