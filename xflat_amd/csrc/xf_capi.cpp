@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <queue>
 #include <vector>
 
 #include "xf_internal.h"
@@ -287,9 +288,7 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     }
     int grid = 0;
     std::vector<int> nblk(E->inst.size()), bpts(E->inst.size()), tiles(E->inst.size());
-    long long total_cells = 0;
     int max_bpt_used = 1;
-    for (auto& in : E->inst) total_cells += in->ncells();
     for (size_t i = 0; i < E->inst.size(); ++i) {
         XfInst* in = E->inst[i].get();
         int ne = in->t.ne;
@@ -311,17 +310,28 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
         if (rc) return rc;
         grid = 0;
         int xrows = 1;
-        for (size_t i = 0; i < E->inst.size(); ++i) {
-            XfInst* in = E->inst[i].get();
-            int nt = tiles[i];
-            if (L.mode != 2) {
-                nblk[i] = nt;
-            } else {
-                // share the co-resident workgroups in proportion to the cells
-                long long share = (long long)maxb * in->ncells() / std::max<long long>(1, total_cells);
-                if (L.grid > 0) share = std::min<long long>(share, (long long)L.grid * in->ncells() / std::max<long long>(1, total_cells));
-                nblk[i] = (int)std::max<long long>(1, std::min<long long>(nt, share));
+        if (L.mode != 2) {
+            for (size_t i = 0; i < E->inst.size(); ++i) nblk[i] = tiles[i];
+        } else {
+            // One workgroup per instance, then each further co-resident
+            // workgroup to the instance whose workgroups carry the most tiles
+            // (the step ends with the busiest workgroup; every tile costs one
+            // pass over the block's cell slots).
+            int budget = maxb;
+            if (L.grid > 0) budget = std::min(budget, L.grid);
+            std::priority_queue<std::pair<int, int>> busiest;     // (tiles per workgroup, instance)
+            for (size_t i = 0; i < E->inst.size(); ++i) {
+                nblk[i] = 1;
+                if (tiles[i] > 1) busiest.push({tiles[i], (int)i});
             }
+            for (int left = budget - (int)E->inst.size(); left > 0 && !busiest.empty(); --left) {
+                const int i = busiest.top().second;
+                busiest.pop();
+                nblk[i] += 1;
+                if (nblk[i] < tiles[i]) busiest.push({(tiles[i] + nblk[i] - 1) / nblk[i], i});
+            }
+        }
+        for (size_t i = 0; i < E->inst.size(); ++i) {
             grid += nblk[i];
             xrows = std::max(xrows, nblk[i]);
         }
