@@ -13,9 +13,10 @@ The headline `value` is the FP32-state stepper: FP32 wave functions and
 propagator increments, FP64 angular moments, error norm, accept/reject and
 unitarity renormalisation.  Reported beside it from the same run: the
 FP64-state stepper ("mixed": FP64 state, FP32 increments), the plain FP64
-stepper, the max |dP_ee| / |dPbar_ee| of each against the FP64 run, and the
-reference's own CPU path (oracle/_ref build, OpenMP over this host's cores)
-on a bounded sample of the same workload.
+stepper, the max |dP_ee| / |dPbar_ee| of each against the FP64 run (after
+the timed steps, and at 20 output radii of an untimed run of the same
+length), and the reference's own CPU path (oracle/_ref build, OpenMP over this
+host's cores) on a bounded sample of the same workload.
 
 Other workloads (one JSON line each, not the headline):
   --workload maa256x64x200   MAA 256 polar x 64 azimuthal x 200 energies
@@ -108,6 +109,37 @@ def time_ctx(params, prec_name, psi0, steps, warmup, tune_steps, launch=None):
     ctx.close()
     return dict(wall_s=t1 - t0, kernel_ms=info.kernel_ms, steps=info.steps, accepted=info.accepted,
                 r=r, dr=dr, psi=psi, launch=li)
+
+
+def accuracy_points(params, prec_name, psi0, total, npts=20):
+    """Untimed accuracy leg: the stepper and the FP64 stepper advance from the
+    same state in npts equal chunks of attempted steps; P_ee, anti-P_ee and the
+    unitarity drift are compared at the end of every chunk (the output radii)."""
+    from xflat_amd import capi
+    a = capi.Context(params, precision=_prec(prec_name))
+    b = capi.Context(params, precision=capi.XF_FP64)
+    try:
+        for c in (a, b):
+            c.upload(psi0, params.R0, DR0)
+        chunk = max(1, total // npts)
+        pts = []
+        done = 0
+        for k in range(npts):
+            n = chunk if k < npts - 1 else max(1, total - done)
+            a.run(n)
+            b.run(n)
+            done += n
+            pa, ra, _ = a.download()
+            pb, rb, _ = b.download()
+            pts.append(dict(_errors(pa, pb), attempts=done, radius_km=ra, radius_km_fp64=rb))
+    finally:
+        a.close()
+        b.close()
+    return {"points": npts, "note": "the stepper vs FP64 from the same state after every 1/%d of the attempted steps" % npts,
+            "max_abs_dPee": max(x["max_abs_dPee"] for x in pts),
+            "max_abs_dPbar_ee": max(x["max_abs_dPbar_ee"] for x in pts),
+            "unitarity_drift": max(x["unitarity_drift"] for x in pts),
+            "per_point": pts}
 
 
 def time_ensemble(params, prec_name, inits, steps, warmup, tune_steps):
@@ -326,6 +358,9 @@ def bench_ma(a, prec_name):
             "after the same number of attempted steps from the same state"), radius_km=[res["r"], ref["r"]],
             same_accepted_count=bool(ref["accepted"] == res["accepted"]))
         print(f"[bench] fp64: {out['fp64']['value']:.3e} beam-steps/s", file=sys.stderr, flush=True)
+        out["pee_error_vs_fp64_at_20_radii"] = accuracy_points(p, prec_name, psi0, a.warmup + a.steps)
+        print(f"[bench] max |dP_ee| vs fp64 over 20 radii: {out['pee_error_vs_fp64_at_20_radii']['max_abs_dPee']:.2e}",
+              file=sys.stderr, flush=True)
         if prec_name == "fp32" and not a.no_extra:
             mx = time_ctx(p, "mixed", psi0, a.steps, a.warmup, a.tune_steps)
             out["fp64_state_mixed"] = dict(
