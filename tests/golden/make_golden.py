@@ -8,7 +8,13 @@ oracle/ref_io_recorder.cpp), on small configurations:
   reference's resume path (fio::fillInitData, src/IO_f.cpp:197-250, which sets
   R0, dr and psi; SA calls it from Phy_SA.cpp:231);
 * MAA cases start from pure flavour states (the reference's MAA resume and
-  perturbation are switched off, Phy_MAA.cpp:281,412).
+  perturbation are switched off, Phy_MAA.cpp:281,412), or, for maa_pert, from
+  the reference's own -DPERTERB perturbation (NBeam.cpp:260-275): each NBeam
+  draws two glibc rand() values per energy bin, nubm_0 first (Nmr.cpp:61),
+  in initBeam's loop order (Phy_MAA.cpp:358-361) on one OpenMP thread; the
+  generator reproduces that state by calling the same libc rand() stream
+  (default seed) in the same order.  The states are not SU(2) partners, so
+  they exercise the 4-species path.
 
 Every accepted step is recorded by the IO hook of Nmr.cpp:238 as
 (it = attempt index, r after the step, dr used); the wave functions are kept
@@ -48,6 +54,8 @@ CASES = {
     # MAA: 4-moment kernel, pure flavour initial states
     "maa_r50": (dict(Abins=8, Pbins=6, Ebins=12, R0=50, Ts=200), None, 0.0, 6),
     "maa_r10": (dict(Abins=8, Pbins=4, Ebins=12, R0=10, Ts=200, dm2="2.4E-3"), None, 0.0, 6),
+    # MAA from the reference's -DPERTERB initial states (glibc rand())
+    "maa_pert": (dict(_module="maa_pert", Abins=8, Pbins=6, Ebins=12, R0=50, Ts=200), None, 0.0, 6),
     # SA: one trajectory (reference built with -DSA), H_vv scaled by D(r)
     "sa_r10_ih": (dict(_module="sa", Abins=1, Ebins=64, R0=10, Ts=400, dm2="-2.4E-3"), 21, 1e-6, 8),
     "sa_r50_nh": (dict(_module="sa", Abins=1, Ebins=100, R0=50, Ts=300, dm2="2.4E-3", eps0="1.E-9"), 22, 1e-6, 8),
@@ -55,12 +63,35 @@ CASES = {
 }
 
 
+def _perterb_psi(nbeam, ne):
+    """The -DPERTERB initial state of NBeam::NBeam(prtc) (NBeam.cpp:246-275)
+    for the first beam set, from libc's rand() with its default seed:
+    eps = (rand() 1e-8 - 0.5e-8) / RAND_MAX, eta likewise, per energy bin;
+    (one_sq, 0, eps, eta) for nu_e / anti-nu_e, (eps, 0, one_sq, eta) else."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)                         # glibc's state before any srand()
+    rand_max = 2147483647.0
+    psi = np.zeros((nbeam, 4, 4, ne))
+    for b in range(nbeam):
+        for n in range(4):
+            for e in range(ne):
+                e1 = libc.rand() * 1.e-8 - (.5 * 1.e-8)
+                e2 = libc.rand() * 1.e-8 - (.5 * 1.e-8)
+                eps, eta = e1 / rand_max, e2 / rand_max
+                one_sq = np.sqrt(1 - eps * eps - eta * eta)
+                psi[b, n, :, e] = (one_sq, 0., eps, eta) if n < 2 else (eps, 0., one_sq, eta)
+    return psi
+
+
 def _run_case(name, over, seed, amp, keep, isa="v3"):
     over = dict(over)
     module = over.pop("_module", None)
     maa = int(over.get("Pbins", 1)) > 1
     sa = module == "sa"
-    exe = os.path.join(REF_DIR, f"xflat_ref_{'sa' if sa else 'maa' if maa else 'ma'}_{isa}")
+    pert = module == "maa_pert"
+    kind = "sa" if sa else "maa_pert" if pert else "maa" if maa else "ma"
+    exe = os.path.join(REF_DIR, f"xflat_ref_{kind}_{'v3' if pert else isa}")
     if not os.path.exists(exe):
         raise SystemExit(f"{exe} missing: run `make -C oracle ref` first")
     txt = config_text(**over)
@@ -69,6 +100,8 @@ def _run_case(name, over, seed, amp, keep, isa="v3"):
     r0 = float(over["R0"]); dr0 = 0.001
     if seed is not None:
         psi0 = seeded_psi(nbeam, ne, seed, amp)
+    elif pert:
+        psi0 = _perterb_psi(nbeam, ne)
     else:
         psi0 = pure_psi(nbeam, ne)
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else None) as td:
@@ -76,7 +109,7 @@ def _run_case(name, over, seed, amp, keep, isa="v3"):
         with open(cfg, "w") as f:
             f.write(txt)
         out = os.path.join(td, "rec.bin")
-        env = dict(os.environ, XFLAT_REF_OUT=out, XFLAT_REF_PSI_EVERY="1", OMP_NUM_THREADS="4")
+        env = dict(os.environ, XFLAT_REF_OUT=out, XFLAT_REF_PSI_EVERY="1", OMP_NUM_THREADS="1" if pert else "4")
         args = [exe, cfg]
         if seed is not None:
             init = os.path.join(td, "init.bin")
