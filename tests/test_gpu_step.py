@@ -373,3 +373,48 @@ def test_seeded_ensemble_fp64_matches_oracle_and_fp32_close():
         assert np.abs(pe - oe).max() <= PEE_TOL and np.abs(pb - ob).max() <= PEE_TOL, i
         (fe, fb) = _pee(res["fp32"][1][i][0])
         assert np.abs(fe - pe).max() <= 1e-5 and np.abs(fb - pb).max() <= 1e-5, i
+
+
+@pytest.mark.parametrize("workload", ["ma1000x400", "maa256x64x200"])
+def test_full_size_fp64_matches_oracle_and_fp32_close(workload):
+    """The bench's full sizes (MA 1000 x 400 from R = 10 km at eps0 = 1e-9;
+    MAA 256 x 64 x 200 with the azimuthal seed) for a few attempted steps:
+    the FP64 stepper takes the oracle's accept/reject and dr sequence with
+    P_ee and Pbar_ee within 1e-9; the FP32 stepper stays within 1e-5."""
+    capi = _capi()
+    from oracle.oracle import Oracle
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import seeded_psi, azimuthal_psi
+    if workload == "ma1000x400":
+        p = bulb_params(Abins=1000, Ebins=400, R0=10, Rn=1000, eps0="1.E-9", dm2="-2.4E-3")
+        psi0 = seeded_psi(1000, 400, seed=2024, amp=1e-6)
+        steps = 100
+    else:
+        p = bulb_params(Abins=256, Pbins=64, Ebins=200, R0=10, Rn=400, Lve="1.25E+51")
+        psi0 = azimuthal_psi(256, 64, 200, seed=31, amp=1e-7)
+        steps = 20
+    print(f"[full size] {workload}: oracle {steps} steps", flush=True)
+    o = Oracle(p)
+    o.set_state(psi0, 10.0, 1e-3)
+    n, orec, secs = o.run(steps)
+    opsi, orr, _, _ = o.state()
+    o.close()
+    print(f"[full size] oracle {secs:.2f} s, {int(orec['accepted'].sum())} accepted", flush=True)
+    res = {}
+    for name, prec in (("fp64", capi.XF_FP64), ("fp32", capi.XF_FP32)):
+        with capi.Context(p, precision=prec) as ctx:
+            ctx.upload(psi0, 10.0, 1e-3)
+            rec = _run_all(ctx, steps)
+            res[name] = (rec, *ctx.download())
+    rec, psi, r, _ = res["fp64"]
+    assert np.array_equal(rec["accepted"], orec["accepted"])
+    np.testing.assert_allclose(rec["dr"], orec["dr"], rtol=DR_RTOL)
+    assert abs(r - orr) <= R_RTOL * orr
+    pe, pb = _pee(psi)
+    oe, ob = _pee(opsi)
+    d64 = max(np.abs(pe - oe).max(), np.abs(pb - ob).max())
+    fe, fb = _pee(res["fp32"][1])
+    d32 = max(np.abs(fe - pe).max(), np.abs(fb - pb).max())
+    print(f"[full size] {workload}: fp64 vs oracle {d64:.2e}, fp32 vs fp64 {d32:.2e}", flush=True)
+    assert d64 <= PEE_TOL
+    assert d32 <= 1e-5
