@@ -33,8 +33,9 @@
 //
 // Precision: FP64 mirrors NBeam::U (NBeam.cpp:360-393) operation by
 // operation.  The FP32 modes propagate increments delta = (U - I) psi in
-// FP32 with the half-angle form cos x - 1 = -2 sin^2(x/2) and hardware
-// sin/cos/rsq; moments, error sums and the decision are FP64.
+// FP32, with cos x - 1 and sin(x)/x - 1 as polynomials in x^2 (prop_f2),
+// and estimate the step error from second-order remainders plus an FP64
+// first-order term (pass C); moments, error sums and the decision are FP64.
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include <cstdint>
@@ -116,22 +117,8 @@ __device__ __forceinline__ Sp<double> apply_d(const PropD& P, const Sp<double>& 
 // (U - I) = [[cm1 + i m1, m2 + i m3], [-m2 + i m3, cm1 - i m1]]
 struct PropF { float cm1, m1, m2, m3; };
 
-__device__ __forceinline__ void prop_f(float h0, float h1, float h2, float dl, PropF& P) {
-    float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
-    float rl = lam2 > 0.f ? __builtin_amdgcn_rsqf(lam2) : 0.f;
-    float lam = lam2 * rl;
-    float x = lam * dl * (float)(0.5 * INV_2PI_D);         // half angle, revolutions
-    x = x - __builtin_rintf(x);
-    float sh = __builtin_amdgcn_sinf(x);
-    float ch = __builtin_amdgcn_cosf(x);
-    float s = 2.f * sh * ch;                               // sin(lam dl)
-    P.cm1 = -2.f * sh * sh;                                // cos(lam dl) - 1
-    float sc = s * rl;
-    P.m1 = -h0 * sc;
-    P.m2 = h2 * sc;
-    P.m3 = -h1 * sc;
-}
-// prop_f plus the coefficients of the remainder R = (U - I) + i dl H, i.e.
+// The FP32 propagator U - I and the coefficients of its remainder
+// R = (U - I) + i dl H, i.e.
 // (cos x - 1) I - i (sin(x)/lambda - dl) H, second order in x = lambda dl.
 // For x^2 < 1 all coefficients come from polynomials in x^2
 // (cos x - 1 = -x^2/2 + x^4/24 - ..., sin(x)/x - 1 = -x^2/6 + x^4/120 - ...):
@@ -996,8 +983,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             if (MODE == 2) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), fin);
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
-                            Sp<float> d1, d3, d4, df;
-                            if constexpr (COMP) {
+                            Sp<float> d3, d4, df;
+                            static_assert(COMP, "FP32 increments always use the compensated error estimate");
+                            {
                                 // increments of the state as below; the error from the
                                 // remainders R = (U - I) + i dl H (second order, so
                                 // rounded relative to x^2) plus the first-order part
@@ -1042,29 +1030,6 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 const Sp<float> E = sp_add(delta_f(Q, a), u);
                                 df = sp_avg(d3n, d4);
                                 dsq = .25 * (double)((E.ar * E.ar + E.ai * E.ai) + (E.br * E.br + E.bi * E.bi));
-                            } else {
-                            if (MODE == 0) {
-                                d1 = C.k1[k][s]; d3 = C.k3[k][s]; d4 = C.k4[k][s];
-                            } else {
-                                double x0, x1, x2;
-                                PropF P1, P0, Pb, P4;
-                                cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);
-                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P1);
-                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 0], P0);
-                                d1 = delta_f(P1, a);
-                                d3 = delta_f(P0, a);
-                                cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);
-                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], Pb);
-                                d1 = sp_avg(delta_f(Pb, a), d1);
-                                cell_h(I, e, s, rec + R::H + 6, x0, x1, x2);
-                                prop_f((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P4);
-                                d4 = delta_f(P4, a);
-                            }
-                            PropF P2;
-                            prop_f((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 2], P2);
-                            Sp<float> d3n = sp_add(d3, delta_f(P2, sp_add(a, d3)));
-                            df = sp_avg(d3n, d4);
-                            dsq = (double)dnorm2<float>(df, d1);
                             }
                             Sp<double> nd = candidate(sp_d(C.p0[k][s]), df, I.renorm);
                             Sp<float> vf = sp_f(nd);
