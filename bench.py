@@ -15,8 +15,9 @@ unitarity renormalisation.  Reported beside it from the same run: the
 FP64-state stepper ("mixed": FP64 state, FP32 increments), the plain FP64
 stepper, the max |dP_ee| / |dPbar_ee| of each against the FP64 run (after
 the timed steps, and at 20 output radii of an untimed run of the same
-length), and the reference's own CPU path (oracle/_ref build, OpenMP over this
-host's cores) on a bounded sample of the same workload.
+length), the same figures in the normal hierarchy, and the reference's own
+CPU path (oracle/_ref build, OpenMP over this host's cores) on a bounded
+sample of the same workload.
 
 Other workloads (one JSON line each, not the headline):
   --workload maa256x64x200   MAA 256 polar x 64 azimuthal x 200 energies
@@ -361,6 +362,18 @@ def bench_ma(a, prec_name):
         out["pee_error_vs_fp64_at_20_radii"] = accuracy_points(p, prec_name, psi0, a.warmup + a.steps)
         print(f"[bench] max |dP_ee| vs fp64 over 20 radii: {out['pee_error_vs_fp64_at_20_radii']['max_abs_dPee']:.2e}",
               file=sys.stderr, flush=True)
+        if not a.no_extra:
+            # the same workload in the normal hierarchy (configs run both)
+            pn = _cfg(**dict(w["params"], dm2="2.4E-3"))
+            rn = time_ctx(pn, prec_name, psi0, a.steps, a.warmup, a.tune_steps, launch)
+            fn = time_ctx(pn, "fp64", psi0, a.steps, a.warmup, a.tune_steps)
+            out["normal_hierarchy"] = dict(
+                {"value": cells * rn["steps"] / rn["wall_s"], "ms_per_step": 1e3 * rn["wall_s"] / max(1, rn["steps"]),
+                 "accepted_steps": rn["accepted"], "radius_km": rn["r"],
+                 "fp64_value": cells * fn["steps"] / fn["wall_s"], "fp64_accepted_steps": fn["accepted"],
+                 "fp64_radius_km": fn["r"]}, **_errors(rn["psi"], fn["psi"]))
+            print(f"[bench] normal hierarchy: {prec_name} {out['normal_hierarchy']['value']:.3e}, fp64 "
+                  f"{out['normal_hierarchy']['fp64_value']:.3e} beam-steps/s", file=sys.stderr, flush=True)
         if prec_name == "fp32" and not a.no_extra:
             mx = time_ctx(p, "mixed", psi0, a.steps, a.warmup, a.tune_steps)
             out["fp64_state_mixed"] = dict(

@@ -375,18 +375,20 @@ def test_seeded_ensemble_fp64_matches_oracle_and_fp32_close():
         assert np.abs(fe - pe).max() <= 1e-5 and np.abs(fb - pb).max() <= 1e-5, i
 
 
-@pytest.mark.parametrize("workload", ["ma1000x400", "maa256x64x200"])
+@pytest.mark.parametrize("workload", ["ma1000x400", "ma1000x400_nh", "maa256x64x200"])
 def test_full_size_fp64_matches_oracle_and_fp32_close(workload):
-    """The bench's full sizes (MA 1000 x 400 from R = 10 km at eps0 = 1e-9;
-    MAA 256 x 64 x 200 with the azimuthal seed) for a few attempted steps:
+    """The bench's full sizes (MA 1000 x 400 from R = 10 km at eps0 = 1e-9 in
+    both hierarchies; MAA 256 x 64 x 200 with the azimuthal seed) for a few
+    attempted steps:
     the FP64 stepper takes the oracle's accept/reject and dr sequence with
     P_ee and Pbar_ee within 1e-9; the FP32 stepper stays within 1e-5."""
     capi = _capi()
     from oracle.oracle import Oracle
     from xflat_amd.config import bulb_params
     from xflat_amd.seed import seeded_psi, azimuthal_psi
-    if workload == "ma1000x400":
-        p = bulb_params(Abins=1000, Ebins=400, R0=10, Rn=1000, eps0="1.E-9", dm2="-2.4E-3")
+    if workload.startswith("ma1000x400"):
+        dm2 = "2.4E-3" if workload.endswith("_nh") else "-2.4E-3"
+        p = bulb_params(Abins=1000, Ebins=400, R0=10, Rn=1000, eps0="1.E-9", dm2=dm2)
         psi0 = seeded_psi(1000, 400, seed=2024, amp=1e-6)
         steps = 100
     else:
