@@ -420,3 +420,41 @@ def test_full_size_fp64_matches_oracle_and_fp32_close(workload):
     print(f"[full size] {workload}: fp64 vs oracle {d64:.2e}, fp32 vs fp64 {d32:.2e}", flush=True)
     assert d64 <= PEE_TOL
     assert d32 <= 1e-5
+
+
+@pytest.mark.parametrize("module", ["ma", "sa"])
+def test_run_ends_exactly_at_rn(module):
+    """The loop ends when r reaches Rn (Nmr.cpp:130); the last step is clamped
+    to Rn - r (Nmr.cpp:249), so the final radius is Rn itself.  The stepper
+    stops there (stop reason 1) after the oracle's number of attempts with its
+    accept sequence; the FP32 stepper lands on Rn too."""
+    capi = _capi()
+    from oracle.oracle import Oracle
+    from xflat_amd.config import bulb_params, XF_SA
+    from xflat_amd.seed import seeded_psi
+    if module == "ma":
+        p = bulb_params(Abins=24, Ebins=16, R0=100, Rn=100.25, eps0="1.E-8")
+        nbeam = 24
+    else:
+        p = bulb_params(module=XF_SA, Abins=1, Ebins=40, R0=100, Rn=100.25, eps0="1.E-8")
+        nbeam = 1
+    psi0 = seeded_psi(nbeam, p.ebins, seed=41, amp=1e-6)
+    o = Oracle(p)
+    o.set_state(psi0, 100.0, 1e-3)
+    n, orec, _ = o.run(5000)
+    opsi, orr, _, _ = o.state()
+    assert orr == p.Rn and n < 5000
+    for prec in (capi.XF_FP64, capi.XF_FP32):
+        with capi.Context(p, precision=prec) as ctx:
+            ctx.upload(psi0, 100.0, 1e-3)
+            info, rec = ctx.run(5000, log=True)
+            psi, r, dr = ctx.download()
+        print(f"[Rn] {module} precision {prec}: {info.steps} attempts (oracle {n}), r = {r!r}", flush=True)
+        assert r == p.Rn
+        assert info.stop_reason == 1
+        if prec == capi.XF_FP64:
+            assert info.steps == n
+            assert np.array_equal(rec["accepted"], orec["accepted"])
+            pe, pb = _pee(psi)
+            oe, ob = _pee(opsi)
+            assert max(np.abs(pe - oe).max(), np.abs(pb - ob).max()) <= PEE_TOL
