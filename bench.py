@@ -112,6 +112,35 @@ def time_ctx(params, prec_name, psi0, steps, warmup, tune_steps, launch=None):
                 r=r, dr=dr, psi=psi, launch=li)
 
 
+def time_short_launches(params, prec_name, psi0, launch, launches=100, per=10):
+    """A radial loop cut into short launches (a host that stops every few
+    steps to test its stop rules or write output): `launches` eager
+    xf_run_steps calls against one xf_run_graph replay of the same launches,
+    each from the same state, after one untimed pass of each."""
+    from xflat_amd import capi
+    ctx = capi.Context(params, precision=_prec(prec_name))
+    ctx.upload(psi0, params.R0, DR0)
+    if launch:
+        mode, cpt, bpt = launch
+        ctx.set_launch(mode, cpt, 64 * (((bpt * params.ebins + cpt - 1) // cpt + 63) // 64), bpt)
+    cells = params.abins * params.ebins
+    out = {"launches": launches, "steps_per_launch": per}
+    for name in ("eager", "graph"):
+        for timed in (False, True):
+            ctx.upload(psi0, params.R0, DR0)
+            t0 = time.perf_counter()
+            if name == "eager":
+                steps = sum(ctx.run(per)[0].steps for _ in range(launches))
+            else:
+                steps = ctx.run_graph(launches, per).steps
+            t1 = time.perf_counter()
+        out[f"{name}_value"] = cells * steps / (t1 - t0)
+        out[f"{name}_ms_per_step"] = 1e3 * (t1 - t0) / max(1, steps)
+        out[f"{name}_radius_km"] = ctx.download()[1]
+    ctx.close()
+    return out
+
+
 def accuracy_points(params, prec_name, psi0, total, npts=20):
     """Untimed accuracy leg: the stepper and the FP64 stepper advance from the
     same state in npts equal chunks of attempted steps; P_ee, anti-P_ee and the
@@ -374,6 +403,11 @@ def bench_ma(a, prec_name):
                  "fp64_radius_km": fn["r"]}, **_errors(rn["psi"], fn["psi"]))
             print(f"[bench] normal hierarchy: {prec_name} {out['normal_hierarchy']['value']:.3e}, fp64 "
                   f"{out['normal_hierarchy']['fp64_value']:.3e} beam-steps/s", file=sys.stderr, flush=True)
+        if not a.no_extra:
+            out["short_launches"] = time_short_launches(p, prec_name, psi0, launch)
+            s = out["short_launches"]
+            print(f"[bench] {s['launches']} launches x {s['steps_per_launch']} steps: eager {s['eager_value']:.3e}, "
+                  f"graph {s['graph_value']:.3e} beam-steps/s", file=sys.stderr, flush=True)
         if prec_name == "fp32" and not a.no_extra:
             mx = time_ctx(p, "mixed", psi0, a.steps, a.warmup, a.tune_steps)
             out["fp64_state_mixed"] = dict(

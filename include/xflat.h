@@ -200,6 +200,13 @@ int  xf_state_species(const xf_ctx* ctx);        /* 2 or 4 in use           */
  * condition fires).  log (optional) receives one record per attempt.       */
 int  xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop,
                   xf_step_rec* log, xf_run_info* info);
+/* The radial loop as a HIP graph: `launches` launches of up to max_steps
+ * attempted steps each, replayed from one instantiated graph (built on first
+ * use, kept while launches, max_steps and the launch shape stay).  The same
+ * steps and results as `launches` calls of xf_run_steps without stop
+ * conditions or log; info counts all of them.  No counterpart in the
+ * reference (the reference's loop is host code, Nmr.cpp:125-252). */
+int  xf_run_graph(xf_ctx* ctx, int launches, int max_steps, xf_run_info* info);
 /* Launch shape in use (after automatic choice). */
 int  xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_tile);
 /* Explicit launch shape: mode 0 keeps psi0 and the step's intermediate
@@ -231,6 +238,8 @@ int  xf_ensemble_upload(xf_ensemble* ens, int i, const double* psi, double r, do
 int  xf_ensemble_init_pure(xf_ensemble* ens, int i);
 int  xf_ensemble_download(xf_ensemble* ens, int i, double* psi, double* r, double* dr);
 int  xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info /*n*/);
+/* xf_run_graph for an ensemble (one graph launches all instances). */
+int  xf_ensemble_run_graph(xf_ensemble* ens, int launches, int max_steps, xf_run_info* info /*n*/);
 /* Launch-shape tuner over all instances (as xf_autotune; states restored). */
 int  xf_ensemble_autotune(xf_ensemble* ens, int steps, xf_opts* best, double* best_ms_per_step);
 int  xf_ensemble_cells(const xf_ensemble* ens, int64_t* cells);

@@ -458,3 +458,55 @@ def test_run_ends_exactly_at_rn(module):
             pe, pb = _pee(psi)
             oe, ob = _pee(opsi)
             assert max(np.abs(pe - oe).max(), np.abs(pb - ob).max()) <= PEE_TOL
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_graph_replays_equal_eager_launches(prec):
+    """xf_run_graph replays one instantiated graph of 4 launches x 25 steps;
+    after every replay (with eager launches and synchronisations of another
+    context in between) the state, radius and step equal those of the same
+    launches run eagerly, bit for bit."""
+    capi = _capi()
+    pr = capi.XF_FP64 if prec == "fp64" else capi.XF_FP32
+    g, p = _golden("ma_r50_ih.npz")
+    with capi.Context(p, precision=pr) as cg, capi.Context(p, precision=pr) as ce:
+        for c in (cg, ce):
+            c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        for rep in range(3):
+            info = cg.run_graph(4, 25)
+            steps = acc = 0
+            for _ in range(4):
+                ie, _ = ce.run(25)
+                steps += ie.steps
+                acc += ie.accepted
+            pg, rg, dg = cg.download()
+            pe, re_, de = ce.download()
+            print(f"[graph] {prec} replay {rep}: {info.steps} steps, {info.accepted} accepted, r = {rg!r}", flush=True)
+            assert (info.steps, info.accepted) == (steps, acc)
+            assert rg == re_ and dg == de
+            assert np.array_equal(pg, pe), f"replay {rep} differs"
+
+
+def test_ensemble_graph_equals_eager_launches():
+    capi = _capi()
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import seeded_psi
+    ps = [bulb_params(Abins=a, Ebins=e, R0=30) for a, e in ((24, 16), (40, 12), (16, 20))]
+    inits = [seeded_psi(p.abins, p.ebins, seed=60 + i, amp=1e-6) for i, p in enumerate(ps)]
+    eg = capi.Ensemble(ps, precision=capi.XF_FP64)
+    ee = capi.Ensemble(ps, precision=capi.XF_FP64)
+    try:
+        for ens in (eg, ee):
+            for i, p in enumerate(ps):
+                ens.upload(i, inits[i], p.R0, 1e-3)
+        for rep in range(2):
+            ig = eg.run_graph(3, 20)
+            ie = [ee.run(20) for _ in range(3)]
+            for i in range(len(ps)):
+                a, ra, da = eg.download(i)
+                b, rb, db = ee.download(i)
+                assert ig[i].steps == sum(x[i].steps for x in ie)
+                assert ra == rb and da == db and np.array_equal(a, b), (rep, i)
+    finally:
+        eg.close()
+        ee.close()

@@ -93,6 +93,7 @@ SIGNATURES = {
     "xf_state_counters": (_I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "xf_state_species": (_I, [_P]),
     "xf_run_steps": (_I, [_P, _I, ctypes.POINTER(XfStop), _P, ctypes.POINTER(XfRunInfo)]),
+    "xf_run_graph": (_I, [_P, _I, _I, ctypes.POINTER(XfRunInfo)]),
     "xf_get_launch": (_I, [_P, _OP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "xf_set_launch": (_I, [_P, _I, _I, _I, _I]),
     "xf_launch_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
@@ -105,6 +106,7 @@ SIGNATURES = {
     "xf_ensemble_init_pure": (_I, [_P, _I]),
     "xf_ensemble_download": (_I, [_P, _I, _DP, _DP, _DP]),
     "xf_ensemble_run_steps": (_I, [_P, _I, ctypes.POINTER(XfRunInfo)]),
+    "xf_ensemble_run_graph": (_I, [_P, _I, _I, ctypes.POINTER(XfRunInfo)]),
     "xf_ensemble_autotune": (_I, [_P, _I, _OP, _DP]),
     "xf_ensemble_cells": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "xf_ensemble_launch": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
@@ -235,6 +237,12 @@ class Context:
             rec = rec[:info.steps]
         return info, rec
 
+    def run_graph(self, launches: int, max_steps: int):
+        """`launches` launches of up to max_steps steps from one HIP graph."""
+        info = XfRunInfo()
+        check(self.L.xf_run_graph(self.h, launches, max_steps, ctypes.byref(info)))
+        return info
+
     def launch(self):
         o = XfOpts(); t = ctypes.c_int(); b = ctypes.c_int()
         check(self.L.xf_get_launch(self.h, ctypes.byref(o), ctypes.byref(t), ctypes.byref(b)))
@@ -350,6 +358,11 @@ class Ensemble:
     def run(self, max_steps):
         infos = (XfRunInfo * self.n)()
         check(self.L.xf_ensemble_run_steps(self.h, max_steps, infos))
+        return list(infos)
+
+    def run_graph(self, launches, max_steps):
+        infos = (XfRunInfo * self.n)()
+        check(self.L.xf_ensemble_run_graph(self.h, launches, max_steps, infos))
         return list(infos)
 
     def cells(self) -> int:

@@ -245,8 +245,16 @@ struct xf_ensemble {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool shaped = false;
+    // the radial loop as an instantiated graph (xf_run_graph), kept while its
+    // key -- launches, steps per launch, launch shape, descriptor arrays -- holds
+    hipGraphExec_t gexec = nullptr;
+    int g_launches = 0, g_steps = 0;
+    XfLaunch g_L{};
+    const XfInstDev* g_insts = nullptr;
+    const int* g_blk = nullptr;
 
     ~xf_ensemble() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (d_insts) (void)hipFree(d_insts);
         if (d_blk_inst) (void)hipFree(d_blk_inst);
         if (d_log) (void)hipFree(d_log);
@@ -450,6 +458,53 @@ static int ens_launch(xf_ensemble* E, int max_steps, xf_step_rec* log, double* m
         if (rs.steps_call > 0)
             HIPCHK(hipMemcpy(log, E->d_log, (size_t)rs.steps_call * sizeof(xf_step_rec), hipMemcpyDeviceToHost));
     }
+    return XF_OK;
+}
+
+// `launches` launches of up to max_steps steps from one instantiated graph:
+// the rows and status are cleared once before it (the graph's own clears keep
+// an error status, so a failed launch ends the rest).  Per-instance run
+// scalars before the graph go to *before.
+static int ens_run_graph(xf_ensemble* E, int launches, int max_steps, std::vector<XfRunScalars>* before, double* ms) {
+    if (launches < 1 || max_steps < 1) return xf_fail(XF_EINVAL, "graph of %d launches x %d steps", launches, max_steps);
+    if (3ll * max_steps + 1 > XF_MAX_EXCHANGES)
+        return xf_fail(XF_EINVAL, "%d steps in one launch exceed the exchange tags", max_steps);
+    if (!E->shaped) {
+        int rc = choose_shape(E);
+        if (rc) return rc;
+    }
+    if (E->inst.size() == 1) {
+        XfInstDev d = E->inst[0]->dev;
+        d.log = nullptr;
+        HIPCHK(hipMemcpy(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice));
+    }
+    before->resize(E->inst.size());
+    for (size_t i = 0; i < E->inst.size(); ++i) {
+        int rc = E->inst[i]->scalars(&(*before)[i]);
+        if (rc) return rc;
+    }
+    if (!E->gexec || E->g_launches != launches || E->g_steps != max_steps ||
+        std::memcmp(&E->g_L, &E->L, sizeof(XfLaunch)) != 0 || E->g_insts != E->d_insts || E->g_blk != E->d_blk_inst) {
+        if (E->gexec) (void)hipGraphExecDestroy(E->gexec);
+        E->gexec = nullptr;
+        int rc = xf_step_build_graph(E->L, E->geom, E->d_insts, E->d_blk_inst, (int)E->inst.size(), max_steps, launches,
+                                     &E->gexec);
+        if (rc) return rc;
+        E->g_launches = launches;
+        E->g_steps = max_steps;
+        E->g_L = E->L;
+        E->g_insts = E->d_insts;
+        E->g_blk = E->d_blk_inst;
+    }
+    int rc = xf_step_prepare(E->d_insts, (int)E->inst.size(), E->stream);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(E->ev0, E->stream));
+    HIPCHK(hipGraphLaunch(E->gexec, E->stream));
+    HIPCHK(hipEventRecord(E->ev1, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, E->ev0, E->ev1));
+    if (ms) *ms = f;
     return XF_OK;
 }
 
@@ -745,6 +800,25 @@ int xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop, xf_step_rec* l
     return fill_info(in, info, ms);
 }
 
+int xf_run_graph(xf_ctx* ctx, int launches, int max_steps, xf_run_info* info) {
+    if (!ctx) return xf_fail(XF_EINVAL, "xf_run_graph: null ctx");
+    xf_ensemble* E = ctx->ens;
+    XfInst* in = E->inst[0].get();
+    in->dev.dump_rstep = 0.;
+    in->dev.dump_last_r = 0.;
+    in->dev.dump_itrstep = 0;
+    std::vector<XfRunScalars> before;
+    double ms = 0;
+    int rc = ens_run_graph(E, launches, max_steps, &before, &ms);
+    if (rc) return rc;
+    rc = fill_info(in, info, ms);
+    if (info) {
+        info->steps = info->total_steps - before[0].total;
+        info->accepted = info->total_accepted - before[0].accepted;
+    }
+    return rc;
+}
+
 int xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_tile) {
     if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
     xf_ensemble* E = ctx->ens;
@@ -881,6 +955,24 @@ int xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info) {
     int first_err = XF_OK;
     for (size_t i = 0; i < ens->inst.size(); ++i) {
         rc = fill_info(ens->inst[i].get(), info ? info + i : nullptr, ms);
+        if (rc && !first_err) first_err = rc;
+    }
+    return first_err;
+}
+int xf_ensemble_run_graph(xf_ensemble* ens, int launches, int max_steps, xf_run_info* info) {
+    if (!ens) return xf_fail(XF_EINVAL, "bad arguments");
+    std::vector<XfRunScalars> before;
+    double ms = 0;
+    int rc = ens_run_graph(ens, launches, max_steps, &before, &ms);
+    if (rc) return rc;
+    int first_err = XF_OK;
+    for (size_t i = 0; i < ens->inst.size(); ++i) {
+        xf_run_info* x = info ? info + i : nullptr;
+        rc = fill_info(ens->inst[i].get(), x, ms);
+        if (x) {
+            x->steps = x->total_steps - before[i].total;
+            x->accepted = x->total_accepted - before[i].accepted;
+        }
         if (rc && !first_err) first_err = rc;
     }
     return first_err;

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <hip/hip_runtime_api.h>
 #include <string>
 #include <vector>
 
@@ -128,3 +129,7 @@ int xf_step_func_info(const XfLaunch& L, int geom, int* vgprs, int* spill_bytes)
 // Clears the run status and the exchange rows of every instance of a launch
 // (a kernel, so the launch sequence holds no memset node).
 int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream);
+// The radial loop as an instantiated graph of `launches` launches of up to
+// max_steps steps (the first without clearing: the caller clears before it).
+int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int ninst,
+                        int max_steps, int launches, hipGraphExec_t* exec);

@@ -94,10 +94,12 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
 }
 
 // Status and exchange rows of every instance to zero (generation tags start
-// at 1, so a cleared word matches no exchange).
-__global__ void k_step_prepare(const XfInstDev* __restrict__ insts) {
+// at 1, so a cleared word matches no exchange).  keep_status leaves a nonzero
+// status in place: the stepper then exits at once, so an error in one launch
+// of a graph ends the later ones and is reported after the graph.
+__global__ void k_step_prepare(const XfInstDev* __restrict__ insts, int keep_status) {
     const XfInstDev& I = insts[blockIdx.y];
-    if (blockIdx.x == 0 && threadIdx.x == 0) I.rs->status = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !keep_status) I.rs->status = 0;
     const long long n = 2ll * XF_ROW_REPLICAS * XF_ROWW * I.nblk;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         I.rows[i] = 0ull;
@@ -106,8 +108,64 @@ __global__ void k_step_prepare(const XfInstDev* __restrict__ insts) {
 int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream) {
     if (ninst <= 0 || ninst > 65535) return xf_fail(XF_EINVAL, "%d instances in one launch", ninst);
     (void)hipGetLastError();    // a launch error, not one left by an earlier handled call
-    hipLaunchKernelGGL(k_step_prepare, dim3(8, ninst), dim3(256), 0, (hipStream_t)stream, d_insts);
+    hipLaunchKernelGGL(k_step_prepare, dim3(8, ninst), dim3(256), 0, (hipStream_t)stream, d_insts, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return xf_fail(XF_EHIP, "prepare launch: %s", hipGetErrorString(e));
+    return XF_OK;
+}
+
+// The radial loop as one graph: the stepper, then launches - 1 pairs (rows
+// cleared with the status kept, stepper), each node after the previous one.
+// The stepper nodes carry the cooperative launch attribute, as
+// hipLaunchCooperativeKernel does for a direct launch.  No memset node: the
+// rows are cleared by a kernel.
+int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int ninst,
+                        int max_steps, int launches, hipGraphExec_t* exec) {
+    XfStepFn f = pick(L, geom);
+    if (!f) return xf_fail(XF_EINVAL, "no stepper for this shape");
+    if (launches < 1 || ninst <= 0 || ninst > 65535) return xf_fail(XF_EINVAL, "graph of %d launches, %d instances", launches, ninst);
+    size_t smem = xf_step_smem_bytes(L, geom);
+    if (smem > XF_LDS_MAX) return xf_fail(XF_EINVAL, "shape needs %zu B of LDS, the workgroup limit is %zu", smem, XF_LDS_MAX);
+    if (smem > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    }
+    (void)hipGetLastError();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipGraphCreate(&g, 0);
+    if (e != hipSuccess) return xf_fail(XF_EHIP, "hipGraphCreate: %s", hipGetErrorString(e));
+    int keep = 1;
+    void* step_args[] = {(void*)&d_insts, (void*)&d_blk_inst, (void*)&max_steps};
+    void* prep_args[] = {(void*)&d_insts, (void*)&keep};
+    hipKernelNodeParams sp{};
+    sp.func = (void*)f;
+    sp.gridDim = dim3(L.grid);
+    sp.blockDim = dim3(L.block);
+    sp.sharedMemBytes = (unsigned)smem;
+    sp.kernelParams = step_args;
+    hipKernelNodeParams pp{};
+    pp.func = (void*)k_step_prepare;
+    pp.gridDim = dim3(8, ninst);
+    pp.blockDim = dim3(256);
+    pp.sharedMemBytes = 0;
+    pp.kernelParams = prep_args;
+    hipLaunchAttributeValue coop{};
+    coop.cooperative = 1;
+    hipGraphNode_t prev = nullptr;
+    for (int k = 0; k < launches && e == hipSuccess; ++k) {
+        hipGraphNode_t node = nullptr;
+        if (k > 0) {
+            e = hipGraphAddKernelNode(&node, g, &prev, 1, &pp);
+            if (e != hipSuccess) break;
+            prev = node;
+        }
+        e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &sp);
+        if (e != hipSuccess) break;
+        e = hipGraphKernelNodeSetAttribute(node, hipKernelNodeAttributeCooperative, &coop);
+        prev = node;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return xf_fail(XF_EHIP, "graph of the radial loop: %s", hipGetErrorString(e));
     return XF_OK;
 }

@@ -672,6 +672,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     const int ne = I.ne, bpt = I.bpt, ntiles = I.ntiles;
     const long long ncells = I.ncells;
     XfRunScalars* rs = I.rs;
+    // an error left by an earlier launch of the same graph: nothing to do
+    // (read before any workgroup of this launch can change it)
+    if (__hip_atomic_load(&rs->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
 
     extern __shared__ double smem[];
     double* s_beam = smem;                                   // [bpt][R::SIZE]
