@@ -597,9 +597,15 @@ static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_m
             L.mode = mode;
             L.cpt = cpt;
             if (!xf_step_has_shape(L, E->geom)) continue;
-            // beams per tile: the eight largest (they decide how evenly the
-            // beams spread over the co-resident workgroups), then halvings
+            // beams per tile: the eight largest that fit the kernel's thread
+            // bound (they decide how evenly the beams spread over the
+            // co-resident workgroups), then halvings
             int bmax = max_bpt(ne_max, cpt);
+            {
+                int vg = 0, spill = 0;
+                const int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
+                while (bmax > 1 && block_for(bmax, ne_max, cpt) > kmax) --bmax;
+            }
             std::vector<int> bpts;
             for (int b = bmax; b >= 1 && b > bmax - 8; --b) bpts.push_back(b);
             for (int b = (bmax - 8) / 2; b >= 1; b /= 2) bpts.push_back(b);
