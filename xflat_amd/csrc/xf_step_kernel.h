@@ -248,74 +248,73 @@ template <int GEOM> struct BRec {
     static constexpr int SIZE = HD + 4;
 };
 
-// Geometry of one beam at r, r+dr/2, r+dr: angle bins (Phy_MA.cpp:172-196,
-// Phy_MAA.cpp:194-210) and path lengths (Phy_MA.cpp:218-227, Phy_MAA.cpp:232-241).
+// Geometry of one beam at one of the step's points r, r+dr/2, r+dr (pt 0,
+// 1, 2): angle bins (Phy_MA.cpp:172-196, Phy_MAA.cpp:194-210), moment
+// weights and the H_vv geometry; the point's cos(theta_R) is stored in the
+// geometry slot, from which beam_dl forms the path lengths.
 // SA (one trajectory): the moment is the source itself (weight 1), the
-// geometry slot holds D(r) = (1 - sqrt(1 - (Rv/r)^2))^2 / 2 (Phy_SA.cpp:250-256)
-// and dl is the sub-step length itself (Phy_SA.cpp:144-147).
+// geometry slot holds D(r) = (1 - sqrt(1 - (Rv/r)^2))^2 / 2 (Phy_SA.cpp:250-256).
 template <int GEOM>
-__device__ void beam_geometry(const XfInstDev& I, int b, double r, double rdr2, double rdr, double dr,
-                              double dr2, double* rec) {
+__device__ __forceinline__ void beam_point(const XfInstDev& I, int b, int pt, double rp, double* rec) {
     using R = BRec<GEOM>;
     if constexpr (GEOM == XF_SA) {
-        const double rp[3] = {r, rdr2, rdr};
-#pragma unroll
-        for (int pt = 0; pt < 3; ++pt) {
-            const double Ra = I.Rv / rp[pt];
-            const double tm = 1. - sqrt(1. - (Ra * Ra));
-            rec[R::W + pt] = 1.;
-            rec[R::G + pt * 3 + 0] = .5 * (tm * tm);
-            rec[R::G + pt * 3 + 1] = 0.;
-            rec[R::G + pt * 3 + 2] = 0.;
-        }
+        const double Ra = I.Rv / rp;
+        const double tm = 1. - sqrt(1. - (Ra * Ra));
+        rec[R::W + pt] = 1.;
+        rec[R::G + pt * 3 + 0] = .5 * (tm * tm);
+        rec[R::G + pt * 3 + 1] = 0.;
+        rec[R::G + pt * 3 + 2] = 0.;
+        return;
+    }
+    const int th = (GEOM == XF_MAA) ? b / I.nph : b;
+    const double x = I.x[th], dx = I.dx[th];
+    if constexpr (GEOM == XF_MAA) {
+        const int ph = b - th * I.nph;
+        const double ra = I.Rv / rp;
+        const double r2 = ra * ra;
+        const double c = sqrt(1. - r2 * (1. - x));
+        const double s = ra * I.sinR[th];
+        const double dc = .5 * (1. / c) * r2 * dx;
+        const double cp = I.cphi[ph], spp = I.sphi[ph];
+        rec[R::W + pt * 4 + 0] = dc;
+        rec[R::W + pt * 4 + 1] = cp * s * dc;
+        rec[R::W + pt * 4 + 2] = spp * s * dc;
+        rec[R::W + pt * 4 + 3] = dc * c;
+        rec[R::G + pt * 3 + 0] = s * cp;
+        rec[R::G + pt * 3 + 1] = s * spp;
+        rec[R::G + pt * 3 + 2] = c;
+    } else {
+        double r2 = I.Rv / rp;
+        r2 *= r2;
+        const double c = sqrt(1. - r2 * (1. - x));
+        const double dc = .5 * (1. / c) * r2 * dx;
+        rec[R::W + pt * 2 + 0] = dc;
+        rec[R::W + pt * 2 + 1] = dc * c;
+        rec[R::G + pt * 3 + 0] = c;
+        rec[R::G + pt * 3 + 1] = 0.;
+        rec[R::G + pt * 3 + 2] = 0.;
+    }
+}
+// Path lengths of the three sub-steps (Phy_MA.cpp:218-227, Phy_MAA.cpp:232-241;
+// SA: the sub-step length itself, Phy_SA.cpp:144-147) from the points'
+// cos(theta_R) that beam_point stored.
+template <int GEOM>
+__device__ __forceinline__ void beam_dl(double* rec, double dr, double dr2) {
+    using R = BRec<GEOM>;
+    if constexpr (GEOM == XF_SA) {
         rec[R::DL + 0] = dr2;
         rec[R::DL + 1] = dr;
         rec[R::DL + 2] = dr2;
-        return;
-    }
-    int th = (GEOM == XF_MAA) ? b / I.nph : b;
-    double x = I.x[th], dx = I.dx[th];
-    double rp[3] = {r, rdr2, rdr};
-    double cs[3];
-#pragma unroll
-    for (int pt = 0; pt < 3; ++pt) {
-        if (GEOM == XF_MAA) {
-            int ph = b - th * I.nph;
-            double ra = I.Rv / rp[pt];
-            double r2 = ra * ra;
-            double c = sqrt(1. - r2 * (1. - x));
-            double s = ra * I.sinR[th];
-            double dc = .5 * (1. / c) * r2 * dx;
-            double cp = I.cphi[ph], spp = I.sphi[ph];
-            rec[R::W + pt * 4 + 0] = dc;
-            rec[R::W + pt * 4 + 1] = cp * s * dc;
-            rec[R::W + pt * 4 + 2] = spp * s * dc;
-            rec[R::W + pt * 4 + 3] = dc * c;
-            rec[R::G + pt * 3 + 0] = s * cp;
-            rec[R::G + pt * 3 + 1] = s * spp;
-            rec[R::G + pt * 3 + 2] = c;
-            cs[pt] = c;
-        } else {
-            double r2 = I.Rv / rp[pt];
-            r2 *= r2;
-            double c = sqrt(1. - r2 * (1. - x));
-            double dc = .5 * (1. / c) * r2 * dx;
-            rec[R::W + pt * 2 + 0] = dc;
-            rec[R::W + pt * 2 + 1] = dc * c;
-            rec[R::G + pt * 3 + 0] = c;
-            rec[R::G + pt * 3 + 1] = 0.;
-            rec[R::G + pt * 3 + 2] = 0.;
-            cs[pt] = c;
-        }
-    }
-    if (GEOM == XF_MAA) {
-        rec[R::DL + 0] = dr2 / ((cs[0] + cs[1]) * .5);
-        rec[R::DL + 1] = dr / ((cs[0] + cs[2]) * .5);
-        rec[R::DL + 2] = dr2 / ((cs[1] + cs[2]) * .5);
+    } else if constexpr (GEOM == XF_MAA) {
+        const double c0 = rec[R::G + 2], c1 = rec[R::G + 5], c2 = rec[R::G + 8];
+        rec[R::DL + 0] = dr2 / ((c0 + c1) * .5);
+        rec[R::DL + 1] = dr / ((c0 + c2) * .5);
+        rec[R::DL + 2] = dr2 / ((c1 + c2) * .5);
     } else {
-        rec[R::DL + 0] = dr2 / (cs[0]);
-        rec[R::DL + 1] = dr / (cs[0]);
-        rec[R::DL + 2] = dr2 / (cs[1]);
+        const double c0 = rec[R::G + 0], c1 = rec[R::G + 3];
+        rec[R::DL + 0] = dr2 / c0;
+        rec[R::DL + 1] = dr / c0;
+        rec[R::DL + 2] = dr2 / c1;
     }
 }
 
@@ -469,12 +468,48 @@ __device__ __forceinline__ double wave_red_dpp(double v) {
     return v;
 }
 // lane 63's value in every lane
-__device__ __forceinline__ double lane63(double v) {
+// Sum (MAX: maximum of values >= 0) of p[i] for i = i0, i0 + 64, ... < n, in
+// that order, with 8 LDS loads in flight instead of one dependent load per
+// term.  The padding terms are +0, which leaves a sum (never -0 here) and a
+// maximum of non-negative values unchanged, so the result has the bits of
+// the one-at-a-time loop.
+template <bool MAX>
+__device__ __forceinline__ double lds_strided(const double* p, int i0, int n) {
+    double x = 0.;
+    for (int i = i0; i < n; i += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = (i + u * 64 < n) ? p[i + u * 64] : 0.;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x = MAX ? fmax(x, v[u]) : x + v[u];
+    }
+    return x;
+}
+// The same over doubles stored as 32-bit halves lo[i], hi[i].
+template <bool MAX>
+__device__ __forceinline__ double lds_strided_halves(const unsigned* lo, const unsigned* hi, int i0, int n) {
+    double x = 0.;
+    for (int i = i0; i < n; i += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = i + u * 64;
+            v[u] = (j < n) ? __longlong_as_double((long long)(((unsigned long long)hi[j] << 32) | lo[j])) : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x = MAX ? fmax(x, v[u]) : x + v[u];
+    }
+    return x;
+}
+
+template <int L>
+__device__ __forceinline__ double lane_bcast(double v) {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, 63);
-    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, L);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), L);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
+__device__ __forceinline__ double lane63(double v) { return lane_bcast<63>(v); }
 
 // Cross-workgroup reduction of NV sums and NMAX maxima (maxima of values
 // >= 0) by an all-gather of tagged rows:
@@ -532,8 +567,7 @@ __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int 
                 const double* src = s_x + (size_t)cc[h] * B;
                 x[h] = 0.;
                 if (cc[h] < nv) {
-                    if (v < NV) for (int i = lane; i < B; i += 64) x[h] += src[i];
-                    else        for (int i = lane; i < B; i += 64) x[h] = fmax(x[h], src[i]);
+                    x[h] = (v < NV) ? lds_strided<false>(src, lane, B) : lds_strided<true>(src, lane, B);
                 }
             }
 #pragma unroll
@@ -616,10 +650,7 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, double* s_x, const XfI
             if (!fail && v < NT) {
                 const unsigned* lo = s_h + (size_t)(2 * v) * nblk;
                 const unsigned* hi = lo + nblk;
-                for (int j = lane; j < nblk; j += 64) {
-                    const double y = __longlong_as_double((long long)(((unsigned long long)hi[j] << 32) | lo[j]));
-                    x[h] = (v < NV) ? x[h] + y : fmax(x[h], y);
-                }
+                x[h] = (v < NV) ? lds_strided_halves<false>(lo, hi, lane, nblk) : lds_strided_halves<true>(lo, hi, lane, nblk);
             }
         }
 #pragma unroll
@@ -736,9 +767,17 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             }
         }
     };
-    // geometry of the tile's beams at the current step; Hb filled separately
-    auto tile_geometry = [&](int t, int nbt, double rr, double rdr2, double rdr, double ddr, double dr2) __attribute__((always_inline)) {
-        for (int b = tid; b < nbt; b += B) beam_geometry<GEOM>(I, t * bpt + b, rr, rdr2, rdr, ddr, dr2, s_beam + b * R::SIZE);
+    // geometry of the tile's beams at the step's three points, one (beam,
+    // point) per thread; the path lengths follow after a barrier (tile_dl, on
+    // the threads that run tile_h), Hb separately
+    auto tile_points = [&](int t, int nbt, double rr, double rdr2, double rdr) __attribute__((always_inline)) {
+        for (int q = tid; q < 3 * nbt; q += B) {
+            const int b = q / 3, pt = q - 3 * b;
+            beam_point<GEOM>(I, t * bpt + b, pt, pt == 0 ? rr : (pt == 1 ? rdr2 : rdr), s_beam + b * R::SIZE);
+        }
+    };
+    auto tile_dl = [&](int nbt, double ddr, double dr2) __attribute__((always_inline)) {
+        for (int b = tid; b < nbt; b += B) beam_dl<GEOM>(s_beam + b * R::SIZE, ddr, dr2);
     };
     auto tile_h = [&](int nbt, int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
         const double coef = F64 ? I.coef : I.coef_f;   // float source weights are pre-scaled
@@ -763,7 +802,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
             if (MODE == 2) load_p0(t, cur);
             __syncthreads();
-            tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+            tile_points(t, nbt, r, rdr2, rdr);
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
@@ -785,6 +824,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 
     long long steps_call = 0, acc_call = 0;
     int stop_reason = 0;
+    double hm_cur = hm_at(I, r);
     while (ok) {
         __syncthreads();
         if (!(steps_call < max_steps)) { stop_reason = 0; break; }
@@ -794,7 +834,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         const double rdr2 = r + dr2;
         long long* tr = (I.trace && lb == 0 && steps_call < I.trace_steps) ? I.trace + steps_call * 16 : nullptr;
         if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
-        const double hm_r = hm_at(I, r), hm_rdr2 = hm_at(I, rdr2), hm_rdr = hm_at(I, rdr);
+        // matter term: at r carried from the previous step (after an accepted
+        // step r is the same double as its r + dr); at r + dr/2 and r + dr
+        // evaluated on lanes 0 and 1 together and broadcast
+        const double hm_r = hm_cur;
+        const double hm_l = hm_at(I, (lane & 1) ? rdr : rdr2);
+        const double hm_rdr2 = lane_bcast<0>(hm_l), hm_rdr = lane_bcast<1>(hm_l);
 
         // =================================================== pass A (S1, S2, S6)
         {
@@ -805,8 +850,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
                 if (MODE == 2) load_p0(t, cur);
                 __syncthreads();
-                tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+                tile_points(t, nbt, r, rdr2, rdr);
                 __syncthreads();
+                tile_dl(nbt, dr, dr2);
                 tile_h(nbt, 0, 0, M0, hm_r);
                 __syncthreads();
 #pragma unroll
@@ -866,9 +912,10 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (MODE == 2) {
                     load_p0(t, cur);
                     __syncthreads();
-                    tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+                    tile_points(t, nbt, r, rdr2, rdr);
                 }
                 __syncthreads();
+                if (MODE == 2) tile_dl(nbt, dr, dr2);
                 tile_h(nbt, 1, 2, M1, hm_rdr);
                 tile_h(nbt, 2, 1, M3, hm_rdr2);
                 __syncthreads();
@@ -928,8 +975,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (MODE == 2) {
                     load_p0(t, cur);
                     __syncthreads();
-                    tile_geometry(t, nbt, r, rdr2, rdr, dr, dr2);
+                    tile_points(t, nbt, r, rdr2, rdr);
                     __syncthreads();
+                    tile_dl(nbt, dr, dr2);
                     tile_h(nbt, 0, 0, M0, hm_r);
                     tile_h(nbt, 1, 2, M1, hm_rdr);
                     tile_h(nbt, 2, 1, M3, hm_rdr2);
@@ -1059,10 +1107,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (tr && tid == 0) tr[14] = (long long)__builtin_amdgcn_s_memrealtime();
                 __syncthreads();
                 for (int b = wave; b < nbt; b += nwaves) {
-                    double sq[4] = {0., 0., 0., 0.};
-                    for (int c = b * ne + lane; c < (b + 1) * ne; c += 64)
+                    double sq[4];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) sq[q] += s_errb[q * B * CPT + c];
+                    for (int q = 0; q < 4; ++q) sq[q] = lds_strided<false>(s_errb + q * B * CPT, b * ne + lane, (b + 1) * ne);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const double w = wave_red_dpp<false>(sq[q]);
@@ -1096,6 +1143,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         const long long it = total;
         if (acc_ok) {
             r += dr;
+            hm_cur = hm_rdr;
             if (MODE == 2) {
                 cur ^= 1;
             } else {
