@@ -587,6 +587,23 @@ __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int 
     if (tr && tid == 0) tr[1] = (long long)__builtin_amdgcn_s_memrealtime();
 }
 
+// Whether word 0 of rows 0 .. n-1 (n <= 256) carries the tag: lane l reads
+// rows l + 64 u for u < 4, four loads in flight and none waiting for
+// another; indexes past the last row read the last row again.
+// Only the tag half (the upper 32 bits, at byte offset 4) is read, through
+// a global address-space pointer.
+__device__ __forceinline__ int rows_current(const unsigned long long* rows, int lane, int n, unsigned tag) {
+    using gtag = const __attribute__((address_space(1))) unsigned int;
+    gtag* t = (gtag*)(reinterpret_cast<const unsigned int*>(rows) + 1);
+    unsigned w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = __hip_atomic_load(t + 2 * min(lane + 64 * u, n - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int mine = 1;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) mine &= w[u] == tag;
+    return mine;
+}
+
 template <int NV, int NMAX>
 __device__ __forceinline__ bool xch_gather(double* s_tot, double* s_x, const XfInstDev& I, int lb, unsigned& gen,
                                            int* status, int* s_flag, long long* tr = nullptr) {
@@ -604,7 +621,7 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, double* s_x, const XfI
         int fail = 0;
         for (int spins = 1;; ++spins) {
             int mine = 1;
-            for (int j = lane; j < nblk; j += 64) mine &= (unsigned)(ld_word(rows + j) >> 32) == tag;
+            for (int j0 = 0; j0 < nblk; j0 += 256) mine &= rows_current(rows + j0, lane, min(nblk - j0, 256), tag);
             if (__all(mine)) break;
             if (wait_expired(t0, spins, status)) { fail = 1; break; }
         }
