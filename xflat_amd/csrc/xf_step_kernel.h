@@ -630,55 +630,64 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, double* s_x, const XfI
     }
     __syncthreads();
     int fail = s_flag[0];
-    // fetch: word p of the value-major array (p = q * nblk + j: half q of
-    // row j), coalesced, XW words per thread in flight; its payload goes to
-    // s_h[p].  A stale tag (a word that had not landed when word 0 had) makes
-    // the whole workgroup fetch again.
-    constexpr int XW = XF_EXCHANGE_WORDS;
-    const int nw = 2 * NT * nblk;
-    unsigned* s_h = reinterpret_cast<unsigned*>(s_x);
+    // fetch and totals: wave w takes values w, w + nwaves, ... two at a
+    // time; lane l reads both halves of rows l, l + 64, l + 128, l + 192 of
+    // each (16 loads in flight per chunk of 256 rows) and adds them in row
+    // order, which is the fixed order of the sum over rows.  A stale tag (a
+    // word that had not landed when word 0 had) makes the wave read the chunk
+    // again.
     const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
-    for (int spins = 1; !fail; ++spins) {
-        int bad = 0;
-        for (int p0 = tid; p0 < nw; p0 += XW * B) {
-            unsigned long long w[XW];
+    int wfail = fail;
+    for (int v0 = wave; v0 < NT; v0 += 2 * nwaves) {
+        const int v1 = v0 + nwaves;
+        const bool two = v1 < NT;                             // wave-uniform
+        double x[2] = {0., 0.};
+        for (int j0 = 0; j0 < nblk && !wfail; j0 += 256) {
+            for (int spins = 1;; ++spins) {
+                unsigned long long w[2][2][4];                // [value][half][row]
 #pragma unroll
-            for (int k = 0; k < XW; ++k) {
-                const int p = p0 + k * B;
-                w[k] = (p < nw) ? ld_word(rows + p) : tagw;
-            }
+                for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int k = 0; k < XW; ++k) {
-                const int p = p0 + k * B;
-                bad |= (unsigned)(w[k] >> 32) != tag;
-                if (p < nw) s_h[p] = (unsigned)w[k];
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int j = j0 + lane + 64 * u;
+                            const int v = h ? v1 : v0;
+                            w[h][q][u] = ((h == 0 || two) && j < nblk) ? ld_word(rows + (size_t)(2 * v + q) * nblk + j) : tagw;
+                        }
+                int bad = 0;
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) bad |= (unsigned)(w[h][q][u] >> 32) != tag;
+                if (!__any(bad)) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const bool is_sum = (h ? v1 : v0) < NV;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (j0 + lane + 64 * u >= nblk) continue;
+                            const double y = __longlong_as_double(
+                                (long long)((w[h][1][u] << 32) | (w[h][0][u] & 0xffffffffull)));
+                            x[h] = is_sum ? x[h] + y : fmax(x[h], y);
+                        }
+                    }
+                    break;
+                }
+                if (wait_expired(t1, spins, status)) { wfail = 1; break; }
             }
         }
-        if (!__syncthreads_or(bad)) break;
-        fail = __syncthreads_or(wait_expired(t1, 16 * spins, status));
+        x[0] = (v0 < NV) ? wave_red_dpp<false>(x[0]) : wave_red_dpp<true>(x[0]);
+        if (lane == 63) s_tot[v0] = x[0];
+        if (two) {
+            x[1] = (v1 < NV) ? wave_red_dpp<false>(x[1]) : wave_red_dpp<true>(x[1]);
+            if (lane == 63) s_tot[v1] = x[1];
+        }
     }
     if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
-    for (int v0 = wave; v0 < NT; v0 += 2 * nwaves) {
-        const int vv[2] = {v0, v0 + nwaves};
-        double x[2] = {0., 0.};
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int v = vv[h];
-            if (!fail && v < NT) {
-                const unsigned* lo = s_h + (size_t)(2 * v) * nblk;
-                const unsigned* hi = lo + nblk;
-                x[h] = (v < NV) ? lds_strided_halves<false>(lo, hi, lane, nblk) : lds_strided_halves<true>(lo, hi, lane, nblk);
-            }
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int v = vv[h];
-            if (v >= NT) break;                       // wave-uniform
-            x[h] = (v < NV) ? wave_red_dpp<false>(x[h]) : wave_red_dpp<true>(x[h]);
-            if (lane == 63) s_tot[v] = x[h];
-        }
-    }
-    __syncthreads();
+    fail = __syncthreads_or(wfail);
     gen += 1u;
     if (tr && tid == 0) tr[3] = (long long)__builtin_amdgcn_s_memrealtime();
     return !fail;
