@@ -67,7 +67,7 @@ struct XfRunScalars {
 // restart at 0); a launch performs at most XF_MAX_EXCHANGES exchanges.
 constexpr int XF_NVMAX = 32;                   // max values per partial row
 constexpr int XF_ROWW = 2 * XF_NVMAX;          // words per row
-constexpr int XF_ROW_REPLICAS = 8;             // copies of every row (reads spread over more lines)
+constexpr int XF_ROW_REPLICAS = 4;             // copies of every row (reads spread over more lines)
 constexpr long long XF_MAX_EXCHANGES = 0xF0000000ll;
 
 struct XfInstDev {

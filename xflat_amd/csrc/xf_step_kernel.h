@@ -520,12 +520,13 @@ __device__ __forceinline__ double lane63(double v) { return lane_bcast<63>(v); }
 //     words 2v (low half) and 2v + 1 (high half), each tagged with gen + 1,
 //     into every replica of rows[gen & 1][replica][word][nblk];
 //  2. wave 0 polls word 0 of all rows of replica lb % XF_ROW_REPLICAS (one
-//     contiguous array) until every row carries the tag;
-//  3. the workgroup fetches all words of that replica (coalesced, 16 words
-//     per lane in flight), checks every tag, stages the payloads in LDS and
-//     sums each value over the rows in a fixed order (one wave per value,
-//     lane l summing rows l, l + 64, ..., then the DPP reduction), which
-//     leaves the same bits in every workgroup.
+//     contiguous array, each lane's loads in flight together) until every
+//     row carries the tag;
+//  3. each wave reads both halves of its values (two at a time) for all rows
+//     of that replica straight into registers (16 loads per lane in flight),
+//     checks every tag (re-reading on a stale one) and sums each value over
+//     the rows in a fixed order (lane l summing rows l, l + 64, ..., then
+//     the DPP reduction), which leaves the same bits in every workgroup.
 // Replicas spread the all-gather's reads, which every workgroup makes of the
 // same few kilobytes, over more memory lines.  The rows are the only data
 // that crosses workgroups, so no fence is needed: each word is one
