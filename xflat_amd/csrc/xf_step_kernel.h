@@ -485,23 +485,6 @@ __device__ __forceinline__ double lds_strided(const double* p, int i0, int n) {
     }
     return x;
 }
-// The same over doubles stored as 32-bit halves lo[i], hi[i].
-template <bool MAX>
-__device__ __forceinline__ double lds_strided_halves(const unsigned* lo, const unsigned* hi, int i0, int n) {
-    double x = 0.;
-    for (int i = i0; i < n; i += 8 * 64) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int j = i + u * 64;
-            v[u] = (j < n) ? __longlong_as_double((long long)(((unsigned long long)hi[j] << 32) | lo[j])) : 0.;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) x = MAX ? fmax(x, v[u]) : x + v[u];
-    }
-    return x;
-}
-
 template <int L>
 __device__ __forceinline__ double lane_bcast(double v) {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
