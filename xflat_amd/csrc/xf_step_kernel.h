@@ -1117,9 +1117,21 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (tr && tid == 0) tr[14] = (long long)__builtin_amdgcn_s_memrealtime();
                 __syncthreads();
                 for (int b = wave; b < nbt; b += nwaves) {
-                    double sq[4];
+                    // lane l sums cells l, l + 64, ... of beam b, in that order, for
+                    // the 4 particles together (16 loads in flight, 4 chains)
+                    double sq[4] = {0., 0., 0., 0.};
+                    const int cend = (b + 1) * ne;
+                    for (int c0 = b * ne + lane; c0 < cend; c0 += 4 * 64) {
+                        double v[4][4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) sq[q] = lds_strided<false>(s_errb + q * B * CPT, b * ne + lane, (b + 1) * ne);
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) v[u][q] = (c0 + u * 64 < cend) ? s_errb[q * B * CPT + c0 + u * 64] : 0.;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) sq[q] += v[u][q];
+                    }
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const double w = wave_red_dpp<false>(sq[q]);
