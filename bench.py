@@ -476,7 +476,7 @@ def bench_ensemble(a, prec_name):
                 })
     print(f"[bench] {prec_name}: {value:.3e} beam-steps/s, launch {res['launch']}", file=sys.stderr, flush=True)
     if not a.no_fp64 and prec_name != "fp64":
-        ens64, ref = time_ensemble(params, "fp64", inits, a.steps, a.warmup, 0)
+        ens64, ref = time_ensemble(params, "fp64", inits, a.steps, a.warmup, a.tune_steps)
         worst = []
         for i in range(96):
             psi, _, _ = ens.download(i)

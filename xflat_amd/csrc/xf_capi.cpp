@@ -606,12 +606,17 @@ static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_m
                 const int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
                 while (bmax > 1 && block_for(bmax, ne_max, cpt) > kmax) --bmax;
             }
-            std::vector<int> bpts;
-            for (int b = bmax; b >= 1 && b > bmax - 8; --b) bpts.push_back(b);
-            for (int b = (bmax - 8) / 2; b >= 1; b /= 2) bpts.push_back(b);
-            for (int bpt : bpts) {
-                L.bpt = bpt;
-                L.block = block_for(bpt, ne_max, cpt);
+            // (bpt cap, block).  An ensemble also tries cap 0: the block that
+            // fits the largest instance's tile, and in it every instance's
+            // largest tile.  A cap sized for the largest NE would leave
+            // instances of small NE with tiles that fill a few lanes.
+            std::vector<std::pair<int, int>> cands;
+            if (E->inst.size() > 1) cands.push_back({0, block_for(bmax, ne_max, cpt)});
+            for (int b = bmax; b >= 1 && b > bmax - 8; --b) cands.push_back({b, block_for(b, ne_max, cpt)});
+            for (int b = (bmax - 8) / 2; b >= 1; b /= 2) cands.push_back({b, block_for(b, ne_max, cpt)});
+            for (const auto& cand : cands) {
+                L.bpt = cand.first;
+                L.block = cand.second;
                 L.grid = 0;
                 int vg = 0, spill = 0;
                 int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
