@@ -510,3 +510,33 @@ def test_ensemble_graph_equals_eager_launches():
     finally:
         eg.close()
         ee.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "fp32"])
+def test_streaming_table_mode_equals_streaming(prec):
+    """Mode 4 keeps the beam records of all of a workgroup's tiles in LDS for
+    the step and runs the arithmetic of mode 2 in the same order: the state,
+    radius and step after 120 steps are bit-identical, with one tile per
+    workgroup and with several (grid capped at 8 workgroups), on MA, MAA and
+    SA."""
+    capi = _capi()
+    from xflat_amd.config import bulb_params
+    pr = {"fp64": capi.XF_FP64, "mixed": capi.XF_MIXED, "fp32": capi.XF_FP32}[prec]
+    cases = [("ma", dict(Abins=200, Ebins=100, R0=10), (2, 2)),
+             ("maa", dict(module=1, Abins=16, Pbins=8, Ebins=40, R0=12), (4, 4)),
+             ("sa", dict(module=2, Abins=1, Ebins=200, R0=12), (2, 1))]
+    for name, kw, (cpt, bpt) in cases:
+        p = bulb_params(**kw)
+        for grid in (0, 8):
+            res = {}
+            for mode in (2, 4):
+                with capi.Context(p, precision=pr, grid=grid) as ctx:
+                    ctx.init_pure()
+                    ctx.set_launch(mode, cpt, 64 * (((bpt * p.ebins + cpt - 1) // cpt + 63) // 64), bpt)
+                    info, _ = ctx.run(120)
+                    res[mode] = (info.steps, info.accepted, *ctx.download(), ctx.launch()["grid"])
+            a, b = res[2], res[4]
+            print(f"[{prec}] {name} grid {a[5]}: {a[0]} steps, r = {a[3]!r}", flush=True)
+            assert a[5] == b[5] and (grid == 0 or a[5] <= grid)
+            assert a[:2] == b[:2] and a[3] == b[3] and a[4] == b[4], (name, grid)
+            assert np.array_equal(a[2], b[2]), (name, grid)

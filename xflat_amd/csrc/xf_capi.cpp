@@ -310,6 +310,7 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     XfLaunch Lq = L;
     Lq.bpt = max_bpt_used;              // LDS per workgroup depends on the largest tile
     Lq.xrows = 1;                       // ... and on the exchange staging (rows of one instance)
+    Lq.wbeams = max_bpt_used;           // ... and in mode 4 on the records of one workgroup's tiles
     int maxb = 0, rc = 0;
     // the staging grows with the workgroup count and may lower the occupancy:
     // settle both (the counts only shrink)
@@ -317,8 +318,8 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
         rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
         if (rc) return rc;
         grid = 0;
-        int xrows = 1;
-        if (L.mode != 2) {
+        int xrows = 1, wbeams = max_bpt_used;
+        if (L.mode != 2 && L.mode != 4) {
             for (size_t i = 0; i < E->inst.size(); ++i) nblk[i] = tiles[i];
         } else {
             // One workgroup per instance, then each further co-resident
@@ -342,15 +343,18 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
         for (size_t i = 0; i < E->inst.size(); ++i) {
             grid += nblk[i];
             xrows = std::max(xrows, nblk[i]);
+            wbeams = std::max(wbeams, (tiles[i] + nblk[i] - 1) / nblk[i] * bpts[i]);
         }
-        if (xrows <= Lq.xrows) break;
-        Lq.xrows = xrows;
+        if (xrows <= Lq.xrows && (L.mode != 4 || wbeams <= Lq.wbeams)) break;
+        Lq.xrows = std::max(Lq.xrows, xrows);
+        if (L.mode == 4) Lq.wbeams = std::max(Lq.wbeams, wbeams);
     }
     rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
     if (rc) return rc;
     if (grid > maxb)
         return xf_fail(XF_EINVAL, "shape needs %d co-resident workgroups, device holds %d", grid, maxb);
     L.xrows = Lq.xrows;
+    L.wbeams = Lq.wbeams;
     if (!commit) {
         L.grid = grid;
         return XF_OK;
@@ -388,6 +392,7 @@ static int choose_shape(xf_ensemble* E) {
     XfLaunch base{};
     base.precision = o.precision;
     base.ns = E->inst[0]->ns;
+    base.grid = o.grid;                 // streaming: at most this many workgroups (0: all co-resident)
     int ne_max = 1;
     for (auto& in : E->inst) ne_max = std::max(ne_max, in->t.ne);
     std::vector<int> modes, cpts;
@@ -589,8 +594,9 @@ static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_m
     bool found = false;
     const XfLaunch prev = E->L;
     const bool had_shape = E->shaped;
-    for (int mode = 0; rc == XF_OK && mode <= 2; ++mode)
+    for (int mode : {0, 1, 2, 4})
         for (int cpt : {1, 2, 4}) {
+            if (rc != XF_OK) break;
             XfLaunch L{};
             L.precision = E->o.precision;
             L.ns = E->inst[0]->ns;
@@ -854,7 +860,8 @@ int xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_ti
 // Set an explicit launch shape (mode: 0 keep, 1 recompute, 2 stream).
 int xf_set_launch(xf_ctx* ctx, int mode, int cpt, int block, int bpt) {
     if (!ctx) return xf_fail(XF_EINVAL, "null ctx");
-    if (mode < 0 || mode > 2) return xf_fail(XF_EINVAL, "mode %d: 0 keep, 1 recompute, 2 stream", mode);
+    if (mode < 0 || mode > 4 || mode == 3)
+        return xf_fail(XF_EINVAL, "mode %d: 0 keep, 1 recompute, 2 stream, 4 stream with the records in LDS", mode);
     if (cpt != 1 && cpt != 2 && cpt != 4) return xf_fail(XF_EINVAL, "cpt %d: 1, 2 or 4", cpt);
     if (block < 64 || block > 1024 || block % 64) return xf_fail(XF_EINVAL, "block %d: a multiple of 64 in [64, 1024]", block);
     if (bpt < 1 || bpt > 128) return xf_fail(XF_EINVAL, "bpt %d: 1..128 beams per tile", bpt);
@@ -866,6 +873,7 @@ int xf_set_launch(xf_ctx* ctx, int mode, int cpt, int block, int bpt) {
     L.cpt = cpt;
     L.block = block;
     L.bpt = bpt;
+    L.grid = E->o.grid;                 // streaming: at most this many workgroups (0: all co-resident)
     if (!xf_step_has_shape(L, E->geom)) return xf_fail(XF_EINVAL, "shape not instantiated");
     return plan_shape(E, L, true);
 }

@@ -111,12 +111,14 @@ struct XfInstDev {
 struct XfLaunch {
     int precision;     // XF_FP64 / XF_MIXED / XF_FP32
     int ns;            // 2 or 4
-    int mode;          // 0 resident-keep, 1 resident-recompute, 2 streaming
+    int mode;          // 0 resident-keep, 1 resident-recompute, 2 streaming, 4 streaming with the
+                       // workgroup's beam records kept in LDS for the step
     int cpt;           // 1, 2, 4
     int block;         // threads per workgroup
     int bpt;           // beams per tile
     int grid;          // workgroups (all instances)
     int xrows;         // largest workgroup count of one instance (exchange staging in LDS)
+    int wbeams;        // mode 4: largest beam count of one workgroup's tiles (records in LDS)
 };
 
 // Launch helpers of the fused stepper (xf_step_host.hip).

@@ -5,11 +5,15 @@
 XfStepFn xf_pick_mixed_sa(int ns, int cpt, int mode) {
     if (ns == 2 && cpt == 1 && mode == 1) return k_step<double, float, 2, XF_SA, 1, 1, 1024>;
     if (ns == 2 && cpt == 1 && mode == 2) return k_step<double, float, 2, XF_SA, 1, 2, 1024>;
+    if (ns == 2 && cpt == 1 && mode == 4) return k_step<double, float, 2, XF_SA, 1, 4, 1024>;
     if (ns == 2 && cpt == 2 && mode == 1) return k_step<double, float, 2, XF_SA, 2, 1, 1024>;
     if (ns == 2 && cpt == 2 && mode == 2) return k_step<double, float, 2, XF_SA, 2, 2, 1024>;
+    if (ns == 2 && cpt == 2 && mode == 4) return k_step<double, float, 2, XF_SA, 2, 4, 1024>;
     if (ns == 2 && cpt == 4 && mode == 1) return k_step<double, float, 2, XF_SA, 4, 1, 512>;
     if (ns == 2 && cpt == 4 && mode == 2) return k_step<double, float, 2, XF_SA, 4, 2, 512>;
+    if (ns == 2 && cpt == 4 && mode == 4) return k_step<double, float, 2, XF_SA, 4, 4, 512>;
     if (ns == 4 && cpt == 1 && mode == 1) return k_step<double, float, 4, XF_SA, 1, 1, 1024>;
     if (ns == 4 && cpt == 1 && mode == 2) return k_step<double, float, 4, XF_SA, 1, 2, 1024>;
+    if (ns == 4 && cpt == 1 && mode == 4) return k_step<double, float, 4, XF_SA, 1, 4, 1024>;
     return nullptr;
 }

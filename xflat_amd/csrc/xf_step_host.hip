@@ -41,7 +41,8 @@ size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
     // the pass-C error buffer doubles as the exchange staging: the lanes'
     // values (in chunks of values) and the fetched rows [2 * 3 * nm][xrows]
     size_t errb = std::max((size_t)4 * L.block * L.cpt, (size_t)6 * nm * std::max(L.xrows, 1));
-    return sizeof(double) * ((size_t)L.bpt * bsize + errb + 4 * (size_t)L.bpt + XF_NVMAX);
+    const size_t recs = L.mode == 4 ? (size_t)std::max(L.wbeams, L.bpt) : (size_t)L.bpt;   // beam records
+    return sizeof(double) * (recs * bsize + errb + 4 * (size_t)L.bpt + XF_NVMAX);
 }
 
 int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {

@@ -29,7 +29,10 @@
 // stored.  Modes: 0 keeps psi0 and the intermediate states in registers for
 // the whole launch (state never touches HBM), 1 keeps only psi0 in registers
 // and recomputes the intermediates in pass C, 2 streams psi0 from HBM in
-// every pass (any size) and writes the accepted candidate to the other buffer.
+// every pass (any size) and writes the accepted candidate to the other buffer,
+// 4 streams like 2 with the beam records of all of the workgroup's tiles kept
+// in LDS for the step (when they fit), so the tiles of a pass need no set-up
+// or barrier of their own.
 //
 // Precision: FP64 mirrors NBeam::U (NBeam.cpp:360-393) operation by
 // operation.  The FP32 modes propagate increments delta = (U - I) psi in
@@ -694,6 +697,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     constexpr bool F64 = std::is_same<CT, double>::value;
     // FP32 increments (mixed and FP32 state): compensated error estimate
     constexpr bool COMP = !F64;
+    // streaming modes: 2 sets up each tile's beam records in every pass; 4
+    // keeps the records of all of the workgroup's tiles in LDS for the step
+    // (geometry once per step, one set-up per pass, no barrier between the
+    // tiles of passes A and B)
+    constexpr bool STREAM = (MODE == 2 || MODE == 4);
+    constexpr bool TABLE = (MODE == 4);
     using R = BRec<GEOM>;
     constexpr int NM = R::NM;
     constexpr int NMV = NM * 3;
@@ -719,7 +728,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 
     extern __shared__ double smem[];
     double* s_beam = smem;                                   // [bpt][R::SIZE]
-    double* s_errb = s_beam + bpt * R::SIZE;                 // [4][B*CPT], between passes the
+    // mode 4: one record per beam of the workgroup's tiles t = lb + j nblk
+    const int ntl = (TABLE && lb < ntiles) ? (ntiles - lb + nblk - 1) / nblk : 1;
+    double* s_errb = s_beam + ntl * bpt * R::SIZE;           // [4][B*CPT], between passes the
                                                              // exchange's staging [2*NMV][nblk]
     const int xcap = max(4 * B * CPT, 2 * NMV * nblk);     // doubles of s_errb
     double* s_errs = s_errb + xcap;                          // [bpt][4]
@@ -789,6 +800,34 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     auto tile_dl = [&](int nbt, double ddr, double dr2) __attribute__((always_inline)) {
         for (int b = tid; b < nbt; b += B) beam_dl<GEOM>(s_beam + b * R::SIZE, ddr, dr2);
     };
+    // mode 4: the same over every beam of the workgroup's tiles (slot q: tile
+    // lb + (q / bpt) nblk, beam q % bpt of it)
+    auto table_beam = [&](int q) __attribute__((always_inline)) -> int {
+        const int j = q / bpt;
+        return (lb + j * nblk) * bpt + (q - j * bpt);
+    };
+    auto table_points = [&](double rr, double rdr2, double rdr) __attribute__((always_inline)) {
+        for (int q = tid; q < 3 * ntl * bpt; q += B) {
+            const int bq = q / 3, pt = q - 3 * bq;
+            const int beam = table_beam(bq);
+            if (beam < I.nbeam)
+                beam_point<GEOM>(I, beam, pt, pt == 0 ? rr : (pt == 1 ? rdr2 : rdr), s_beam + bq * R::SIZE);
+        }
+    };
+    auto table_dl = [&](double ddr, double dr2) __attribute__((always_inline)) {
+        for (int q = tid; q < ntl * bpt; q += B)
+            if (table_beam(q) < I.nbeam) beam_dl<GEOM>(s_beam + q * R::SIZE, ddr, dr2);
+    };
+    auto table_h = [&](int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
+        const double coef = F64 ? I.coef : I.coef_f;
+        for (int q = tid; q < ntl * bpt; q += B)
+            if (table_beam(q) < I.nbeam)
+                beam_h<GEOM>(s_beam + q * R::SIZE, pt, M, coef, hm, s_beam + q * R::SIZE + R::H + slot * 3);
+    };
+    auto table_hd = [&]() __attribute__((always_inline)) {
+        for (int q = tid; q < ntl * bpt; q += B)
+            if (table_beam(q) < I.nbeam) beam_hd<GEOM>(s_beam + q * R::SIZE);
+    };
     auto tile_h = [&](int nbt, int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
         const double coef = F64 ? I.coef : I.coef_f;   // float source weights are pre-scaled
         for (int b = tid; b < nbt; b += B) beam_h<GEOM>(s_beam + b * R::SIZE, pt, M, coef, hm, s_beam + b * R::SIZE + R::H + slot * 3);
@@ -796,7 +835,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 
     const int my_tile0 = lb;                 // resident: exactly this tile
     int nbt_res = 0;
-    if (MODE != 2) {
+    if (!STREAM) {
         nbt_res = tile_cells(my_tile0);
         load_p0(my_tile0, cur);
     }
@@ -808,16 +847,25 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 #pragma unroll
         for (int v = 0; v < NMV; ++v) acc[v] = 0.;
         double dr2 = .5 * dr, rdr = r + dr, rdr2 = r + dr2;
-        for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
-            int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
-            if (MODE == 2) load_p0(t, cur);
+        if constexpr (TABLE) {
+            table_points(r, rdr2, rdr);
             __syncthreads();
-            tile_points(t, nbt, r, rdr2, rdr);
-            __syncthreads();
+        }
+        for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
+            int nbt = STREAM ? tile_cells(t) : nbt_res;
+                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+            if constexpr (TABLE) {
+                load_p0(t, cur);
+            } else {
+                __syncthreads();
+                if (MODE == 2) load_p0(t, cur);      // in flight during the set-up
+                tile_points(t, nbt, r, rdr2, rdr);
+                __syncthreads();
+            }
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
                 if (C.bl[k] < 0) continue;
-                const double* rec = s_beam + C.bl[k] * R::SIZE;
+                const double* rec = tb + C.bl[k] * R::SIZE;
                 CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
@@ -856,19 +904,31 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             double acc[XF_NVMAX];
 #pragma unroll
             for (int v = 0; v < 2 * NMV; ++v) acc[v] = 0.;
-            for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
-                int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
-                if (MODE == 2) load_p0(t, cur);
+            if constexpr (TABLE) {
+                table_points(r, rdr2, rdr);
                 __syncthreads();
-                tile_points(t, nbt, r, rdr2, rdr);
+                table_dl(dr, dr2);
+                table_h(0, 0, M0, hm_r);
                 __syncthreads();
-                tile_dl(nbt, dr, dr2);
-                tile_h(nbt, 0, 0, M0, hm_r);
-                __syncthreads();
+            }
+            for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
+                int nbt = STREAM ? tile_cells(t) : nbt_res;
+                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+                if constexpr (TABLE) {
+                    load_p0(t, cur);
+                } else {
+                    __syncthreads();                      // the previous tile is done with s_beam
+                    if (MODE == 2) load_p0(t, cur);      // in flight during the set-up
+                    tile_points(t, nbt, r, rdr2, rdr);
+                    __syncthreads();
+                    tile_dl(nbt, dr, dr2);
+                    tile_h(nbt, 0, 0, M0, hm_r);
+                    __syncthreads();
+                }
 #pragma unroll
                 for (int k = 0; k < CPT; ++k) {
                     if (C.bl[k] < 0) continue;
-                    const double* rec = s_beam + C.bl[k] * R::SIZE;
+                    const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0), Y0 = CT(0), Y2 = CT(0), Y3 = CT(0);
 #pragma unroll
@@ -917,22 +977,32 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             double acc[XF_NVMAX];
 #pragma unroll
             for (int v = 0; v < NMV; ++v) acc[v] = 0.;
-            for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
-                int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
-                if (MODE == 2) {
+            if constexpr (TABLE) {
+                table_h(1, 2, M1, hm_rdr);
+                table_h(2, 1, M3, hm_rdr2);
+                __syncthreads();
+            }
+            for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
+                int nbt = STREAM ? tile_cells(t) : nbt_res;
+                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+                if constexpr (TABLE) {
                     load_p0(t, cur);
+                } else {
+                    if (MODE == 2) {
+                        __syncthreads();
+                        load_p0(t, cur);                  // in flight during the set-up
+                        tile_points(t, nbt, r, rdr2, rdr);
+                    }
                     __syncthreads();
-                    tile_points(t, nbt, r, rdr2, rdr);
+                    if (MODE == 2) tile_dl(nbt, dr, dr2);
+                    tile_h(nbt, 1, 2, M1, hm_rdr);
+                    tile_h(nbt, 2, 1, M3, hm_rdr2);
+                    __syncthreads();
                 }
-                __syncthreads();
-                if (MODE == 2) tile_dl(nbt, dr, dr2);
-                tile_h(nbt, 1, 2, M1, hm_rdr);
-                tile_h(nbt, 2, 1, M3, hm_rdr2);
-                __syncthreads();
 #pragma unroll
                 for (int k = 0; k < CPT; ++k) {
                     if (C.bl[k] < 0) continue;
-                    const double* rec = s_beam + C.bl[k] * R::SIZE;
+                    const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
@@ -980,23 +1050,33 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 #pragma unroll
             for (int v = 0; v < NMV + 1; ++v) acc[v] = 0.;
             double blk_err = 0.;
-            for (int t = (MODE == 2 ? lb : my_tile0); t < (MODE == 2 ? ntiles : my_tile0 + 1); t += (MODE == 2 ? nblk : 1)) {
-                int nbt = (MODE == 2) ? tile_cells(t) : nbt_res;
-                if (MODE == 2) {
+            if constexpr (TABLE) {
+                table_h(3, 2, M4, hm_rdr);
+                if constexpr (COMP) table_hd();
+                __syncthreads();
+            }
+            for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
+                int nbt = STREAM ? tile_cells(t) : nbt_res;
+                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+                if constexpr (TABLE) {
                     load_p0(t, cur);
+                } else {
+                    if (MODE == 2) {
+                        __syncthreads();
+                        load_p0(t, cur);                  // in flight during the set-up
+                        tile_points(t, nbt, r, rdr2, rdr);
+                        __syncthreads();
+                        tile_dl(nbt, dr, dr2);
+                        tile_h(nbt, 0, 0, M0, hm_r);
+                        tile_h(nbt, 1, 2, M1, hm_rdr);
+                        tile_h(nbt, 2, 1, M3, hm_rdr2);
+                    }
                     __syncthreads();
-                    tile_points(t, nbt, r, rdr2, rdr);
+                    tile_h(nbt, 3, 2, M4, hm_rdr);
+                    if constexpr (COMP)
+                        for (int b = tid; b < nbt; b += B) beam_hd<GEOM>(s_beam + b * R::SIZE);
                     __syncthreads();
-                    tile_dl(nbt, dr, dr2);
-                    tile_h(nbt, 0, 0, M0, hm_r);
-                    tile_h(nbt, 1, 2, M1, hm_rdr);
-                    tile_h(nbt, 2, 1, M3, hm_rdr2);
                 }
-                __syncthreads();
-                tile_h(nbt, 3, 2, M4, hm_rdr);
-                if constexpr (COMP)
-                    for (int b = tid; b < nbt; b += B) beam_hd<GEOM>(s_beam + b * R::SIZE);
-                __syncthreads();
 #pragma unroll
                 for (int k = 0; k < CPT; ++k) {
                     const int slot = k * B + tid;
@@ -1005,7 +1085,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         for (int q = 0; q < 4; ++q) s_errb[q * B * CPT + slot] = 0.;
                         continue;
                     }
-                    const double* rec = s_beam + C.bl[k] * R::SIZE;
+                    const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
@@ -1040,8 +1120,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             Sp<double> fin = sp_avg(q3n, q4);
                             dsq = dnorm2<double>(fin, q1);
                             src_add<CT>(I, e, s, fin, X0, X2, X3);
-                            if (MODE != 2) C.k4[k][s] = fin;
-                            if (MODE == 2) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), fin);
+                            if (!STREAM) C.k4[k][s] = fin;
+                            if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), fin);
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
                             Sp<float> d3, d4, df;
@@ -1096,11 +1176,11 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             Sp<float> vf = sp_f(nd);
                             src_add<CT>(I, e, s, vf, X0, X2, X3);
                             if constexpr (std::is_same<ST, float>::value) {
-                                if (MODE != 2) C.k4[k][s] = vf;        // the candidate state itself
+                                if (!STREAM) C.k4[k][s] = vf;        // the candidate state itself
                             } else {
-                                if (MODE != 2) C.k4[k][s] = df;        // increment; re-formed on accept
+                                if (!STREAM) C.k4[k][s] = df;        // increment; re-formed on accept
                             }
-                            if (MODE == 2) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), nd);
+                            if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), nd);
                         }
                         // error weights of the 4 particles (NBeam.cpp:922): with 2 stored
                         // species nu_x / anti-nu_x share |dpsi|^2 with nu_e / anti-nu_e
@@ -1166,7 +1246,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         if (acc_ok) {
             r += dr;
             hm_cur = hm_rdr;
-            if (MODE == 2) {
+            if (STREAM) {
                 cur ^= 1;
             } else {
 #pragma unroll
@@ -1202,7 +1282,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     if (!ok) stop_reason = 3;
 
     // ---- write back
-    if (MODE != 2 && my_tile0 < ntiles) {
+    if (!STREAM && my_tile0 < ntiles) {
         ST* base = (ST*)I.psi[cur];
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
