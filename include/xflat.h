@@ -211,9 +211,9 @@ int  xf_run_graph(xf_ctx* ctx, int launches, int max_steps, xf_run_info* info);
 int  xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_tile);
 /* Explicit launch shape: mode 0 keeps psi0 and the step's intermediate
  * states in registers for the whole launch, 1 keeps psi0 only and recomputes,
- * 2 streams the state from HBM, 4 streams it with the beam records of all of
- * a workgroup's tiles kept in LDS for the step (XF_EINVAL where they do not
- * fit); cpt energy cells per thread; block threads; bpt beams per tile; a
+ * 2 streams the state from HBM, 4 streams it with the beam records of a
+ * workgroup's tiles kept in LDS (for the whole step when they fit, else in
+ * chunks); cpt energy cells per thread; block threads; bpt beams per tile; a
  * streaming grid is capped by xf_opts.grid.  No counterpart in the reference
  * (a GPU launch knob). */
 int  xf_set_launch(xf_ctx* ctx, int mode, int cpt, int block, int bpt);

@@ -514,20 +514,24 @@ def test_ensemble_graph_equals_eager_launches():
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed", "fp32"])
 def test_streaming_table_mode_equals_streaming(prec):
-    """Mode 4 keeps the beam records of all of a workgroup's tiles in LDS for
-    the step and runs the arithmetic of mode 2 in the same order: the state,
-    radius and step after 120 steps are bit-identical, with one tile per
-    workgroup and with several (grid capped at 8 workgroups), on MA, MAA and
+    """Mode 4 keeps the beam records of a workgroup's tiles in LDS (all of
+    them for the step when they fit, else in chunks) and runs the arithmetic
+    of mode 2 in the same order: the state, radius and step after 120 steps
+    are bit-identical, with one tile per workgroup, with several (grid capped
+    at 8 workgroups), and with more records than LDS holds, on MA, MAA and
     SA."""
     capi = _capi()
     from xflat_amd.config import bulb_params
     pr = {"fp64": capi.XF_FP64, "mixed": capi.XF_MIXED, "fp32": capi.XF_FP32}[prec]
-    cases = [("ma", dict(Abins=200, Ebins=100, R0=10), (2, 2)),
-             ("maa", dict(module=1, Abins=16, Pbins=8, Ebins=40, R0=12), (4, 4)),
-             ("sa", dict(module=2, Abins=1, Ebins=200, R0=12), (2, 1))]
-    for name, kw, (cpt, bpt) in cases:
+    cases = [("ma", dict(Abins=200, Ebins=100, R0=10), (2, 2), (0, 8)),
+             ("maa", dict(module=1, Abins=16, Pbins=8, Ebins=40, R0=12), (4, 4), (0, 8)),
+             ("sa", dict(module=2, Abins=1, Ebins=200, R0=12), (2, 1), (0, 8)),
+             # 16 tiles of 64 beams per workgroup: more records than LDS
+             # holds, so mode 4 takes them in chunks
+             ("ma chunked", dict(Abins=4000, Ebins=8, R0=10), (1, 64), (4,))]
+    for name, kw, (cpt, bpt), grids in cases:
         p = bulb_params(**kw)
-        for grid in (0, 8):
+        for grid in grids:
             res = {}
             for mode in (2, 4):
                 with capi.Context(p, precision=pr, grid=grid) as ctx:

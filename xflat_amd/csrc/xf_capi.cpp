@@ -345,6 +345,16 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
             xrows = std::max(xrows, nblk[i]);
             wbeams = std::max(wbeams, (tiles[i] + nblk[i] - 1) / nblk[i] * bpts[i]);
         }
+        if (L.mode == 4) {
+            // the records of one workgroup's tiles, as many as LDS holds (the
+            // kernel takes its tiles in chunks of wbeams / bpt)
+            XfLaunch Lr = Lq;
+            Lr.xrows = std::max(Lq.xrows, xrows);
+            const int cap = xf_step_max_records(Lr, E->geom);
+            if (cap < max_bpt_used)
+                return xf_fail(XF_EINVAL, "mode 4: LDS holds %d beam records, a tile has %d", cap, max_bpt_used);
+            wbeams = std::min(wbeams, cap);
+        }
         if (xrows <= Lq.xrows && (L.mode != 4 || wbeams <= Lq.wbeams)) break;
         Lq.xrows = std::max(Lq.xrows, xrows);
         if (L.mode == 4) Lq.wbeams = std::max(Lq.wbeams, wbeams);
@@ -368,6 +378,7 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
         if (rc) return rc;
         in->dev.bpt = bpts[i];
         in->dev.ntiles = tiles[i];
+        in->dev.tchunk = L.mode == 4 ? std::max(1, L.wbeams / bpts[i]) : 1;
         in->dev.blk0 = b0;
         in->dev.nblk = nblk[i];
         for (int j = 0; j < nblk[i]; ++j) blk_inst[b0 + j] = (int)i;

@@ -104,7 +104,7 @@ struct XfInstDev {
     // stop predicates
     double dump_rstep, dump_last_r;
     int dump_itrstep;
-    int pad;
+    int tchunk;                    // mode 4: tiles whose beam records LDS holds at once
 };
 
 // ---- launch configuration ---------------------------------------------------
@@ -124,6 +124,7 @@ struct XfLaunch {
 // Launch helpers of the fused stepper (xf_step_host.hip).
 bool xf_step_has_shape(const XfLaunch& L, int geom);
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom);
+int xf_step_max_records(const XfLaunch& L, int geom);
 int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks);
 int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int max_steps,
                         void* stream);

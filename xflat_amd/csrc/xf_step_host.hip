@@ -35,14 +35,31 @@ static XfStepFn pick(const XfLaunch& L, int geom) {
 
 bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nullptr; }
 
+// Doubles per beam record in LDS (at least the kernel's BRec size).
+static int record_doubles(int geom) {
+    const int nm = geom == XF_MAA ? 4 : geom == XF_SA ? 1 : 2;
+    return 3 + 3 * nm + 9 + 12 + 4;
+}
+
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
     int nm = geom == XF_MAA ? 4 : geom == XF_SA ? 1 : 2;
-    int bsize = 3 + 3 * nm + 9 + 12 + 4;
+    int bsize = record_doubles(geom);
     // the pass-C error buffer doubles as the exchange staging: the lanes'
     // values (in chunks of values) and the fetched rows [2 * 3 * nm][xrows]
     size_t errb = std::max((size_t)4 * L.block * L.cpt, (size_t)6 * nm * std::max(L.xrows, 1));
     const size_t recs = L.mode == 4 ? (size_t)std::max(L.wbeams, L.bpt) : (size_t)L.bpt;   // beam records
     return sizeof(double) * (recs * bsize + errb + 4 * (size_t)L.bpt + XF_NVMAX);
+}
+
+// Mode 4: the most beam records that fit the workgroup's LDS beside the rest
+// of the layout, with room for the kernel's static LDS.
+int xf_step_max_records(const XfLaunch& L, int geom) {
+    XfLaunch Z = L;
+    Z.wbeams = 0;                               // records = bpt
+    const size_t rec = sizeof(double) * (size_t)record_doubles(geom);
+    const size_t rest = xf_step_smem_bytes(Z, geom) - (size_t)Z.bpt * rec;
+    const size_t budget = XF_LDS_MAX - 2048;
+    return rest >= budget ? 0 : (int)((budget - rest) / rec);
 }
 
 int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {

@@ -30,9 +30,9 @@
 // the whole launch (state never touches HBM), 1 keeps only psi0 in registers
 // and recomputes the intermediates in pass C, 2 streams psi0 from HBM in
 // every pass (any size) and writes the accepted candidate to the other buffer,
-// 4 streams like 2 with the beam records of all of the workgroup's tiles kept
-// in LDS for the step (when they fit), so the tiles of a pass need no set-up
-// or barrier of their own.
+// 4 streams like 2 with the beam records of the workgroup's tiles kept in
+// LDS (for the whole step when they fit, else in chunks of tiles), so the
+// tiles of a chunk need no set-up or barrier of their own.
 //
 // Precision: FP64 mirrors NBeam::U (NBeam.cpp:360-393) operation by
 // operation.  The FP32 modes propagate increments delta = (U - I) psi in
@@ -728,9 +728,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 
     extern __shared__ double smem[];
     double* s_beam = smem;                                   // [bpt][R::SIZE]
-    // mode 4: one record per beam of the workgroup's tiles t = lb + j nblk
+    // mode 4: one record per beam of the workgroup's tiles t = lb + j nblk,
+    // tch tiles at a time (all of them when they fit: nch == 1)
     const int ntl = (TABLE && lb < ntiles) ? (ntiles - lb + nblk - 1) / nblk : 1;
-    double* s_errb = s_beam + ntl * bpt * R::SIZE;           // [4][B*CPT], between passes the
+    const int tch = TABLE ? max(1, min(I.tchunk, ntl)) : 1;
+    const bool chunked = TABLE && tch < ntl;                  // workgroup-uniform
+    double* s_errb = s_beam + tch * bpt * R::SIZE;           // [4][B*CPT], between passes the
                                                              // exchange's staging [2*NMV][nblk]
     const int xcap = max(4 * B * CPT, 2 * NMV * nblk);     // doubles of s_errb
     double* s_errs = s_errb + xcap;                          // [bpt][4]
@@ -802,31 +805,39 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     };
     // mode 4: the same over every beam of the workgroup's tiles (slot q: tile
     // lb + (q / bpt) nblk, beam q % bpt of it)
-    auto table_beam = [&](int q) __attribute__((always_inline)) -> int {
+    // (slot q of the chunk that starts at local tile c0: tile
+    // lb + (c0 + q / bpt) nblk, beam q % bpt of it; the same thread forms a
+    // slot's path lengths, Hamiltonians and Hd, so they need no barrier
+    // between them)
+    auto table_beam = [&](int c0, int q) __attribute__((always_inline)) -> int {
         const int j = q / bpt;
-        return (lb + j * nblk) * bpt + (q - j * bpt);
+        return (lb + (c0 + j) * nblk) * bpt + (q - j * bpt);
     };
-    auto table_points = [&](double rr, double rdr2, double rdr) __attribute__((always_inline)) {
-        for (int q = tid; q < 3 * ntl * bpt; q += B) {
+    auto table_points = [&](int c0, double rr, double rdr2, double rdr) __attribute__((always_inline)) {
+        const int nq = min(tch, ntl - c0) * bpt;
+        for (int q = tid; q < 3 * nq; q += B) {
             const int bq = q / 3, pt = q - 3 * bq;
-            const int beam = table_beam(bq);
+            const int beam = table_beam(c0, bq);
             if (beam < I.nbeam)
                 beam_point<GEOM>(I, beam, pt, pt == 0 ? rr : (pt == 1 ? rdr2 : rdr), s_beam + bq * R::SIZE);
         }
     };
-    auto table_dl = [&](double ddr, double dr2) __attribute__((always_inline)) {
-        for (int q = tid; q < ntl * bpt; q += B)
-            if (table_beam(q) < I.nbeam) beam_dl<GEOM>(s_beam + q * R::SIZE, ddr, dr2);
+    auto table_dl = [&](int c0, double ddr, double dr2) __attribute__((always_inline)) {
+        const int nq = min(tch, ntl - c0) * bpt;
+        for (int q = tid; q < nq; q += B)
+            if (table_beam(c0, q) < I.nbeam) beam_dl<GEOM>(s_beam + q * R::SIZE, ddr, dr2);
     };
-    auto table_h = [&](int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
+    auto table_h = [&](int c0, int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
         const double coef = F64 ? I.coef : I.coef_f;
-        for (int q = tid; q < ntl * bpt; q += B)
-            if (table_beam(q) < I.nbeam)
+        const int nq = min(tch, ntl - c0) * bpt;
+        for (int q = tid; q < nq; q += B)
+            if (table_beam(c0, q) < I.nbeam)
                 beam_h<GEOM>(s_beam + q * R::SIZE, pt, M, coef, hm, s_beam + q * R::SIZE + R::H + slot * 3);
     };
-    auto table_hd = [&]() __attribute__((always_inline)) {
-        for (int q = tid; q < ntl * bpt; q += B)
-            if (table_beam(q) < I.nbeam) beam_hd<GEOM>(s_beam + q * R::SIZE);
+    auto table_hd = [&](int c0) __attribute__((always_inline)) {
+        const int nq = min(tch, ntl - c0) * bpt;
+        for (int q = tid; q < nq; q += B)
+            if (table_beam(c0, q) < I.nbeam) beam_hd<GEOM>(s_beam + q * R::SIZE);
     };
     auto tile_h = [&](int nbt, int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
         const double coef = F64 ? I.coef : I.coef_f;   // float source weights are pre-scaled
@@ -847,14 +858,16 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 #pragma unroll
         for (int v = 0; v < NMV; ++v) acc[v] = 0.;
         double dr2 = .5 * dr, rdr = r + dr, rdr2 = r + dr2;
-        if constexpr (TABLE) {
-            table_points(r, rdr2, rdr);
-            __syncthreads();
-        }
         for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
             int nbt = STREAM ? tile_cells(t) : nbt_res;
-                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+                const int jt = TABLE ? (t - lb) / nblk : 0;           // local tile index
+                const double* tb = TABLE ? s_beam + (jt % tch) * bpt * R::SIZE : s_beam;
             if constexpr (TABLE) {
+                if (jt % tch == 0) {                  // a chunk starts: its records
+                    if (jt > 0) __syncthreads();
+                    table_points(jt, r, rdr2, rdr);
+                    __syncthreads();
+                }
                 load_p0(t, cur);
             } else {
                 __syncthreads();
@@ -904,17 +917,19 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             double acc[XF_NVMAX];
 #pragma unroll
             for (int v = 0; v < 2 * NMV; ++v) acc[v] = 0.;
-            if constexpr (TABLE) {
-                table_points(r, rdr2, rdr);
-                __syncthreads();
-                table_dl(dr, dr2);
-                table_h(0, 0, M0, hm_r);
-                __syncthreads();
-            }
             for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
                 int nbt = STREAM ? tile_cells(t) : nbt_res;
-                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+                const int jt = TABLE ? (t - lb) / nblk : 0;           // local tile index
+                const double* tb = TABLE ? s_beam + (jt % tch) * bpt * R::SIZE : s_beam;
                 if constexpr (TABLE) {
+                    if (jt % tch == 0) {                  // a chunk starts: its records
+                        if (jt > 0) __syncthreads();
+                        table_points(jt, r, rdr2, rdr);
+                        __syncthreads();
+                        table_dl(jt, dr, dr2);
+                        table_h(jt, 0, 0, M0, hm_r);
+                        __syncthreads();
+                    }
                     load_p0(t, cur);
                 } else {
                     __syncthreads();                      // the previous tile is done with s_beam
@@ -977,15 +992,25 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             double acc[XF_NVMAX];
 #pragma unroll
             for (int v = 0; v < NMV; ++v) acc[v] = 0.;
-            if constexpr (TABLE) {
-                table_h(1, 2, M1, hm_rdr);
-                table_h(2, 1, M3, hm_rdr2);
+            if (TABLE && !chunked) {                  // the records of pass A stay
+                table_h(0, 1, 2, M1, hm_rdr);
+                table_h(0, 2, 1, M3, hm_rdr2);
                 __syncthreads();
             }
             for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
                 int nbt = STREAM ? tile_cells(t) : nbt_res;
-                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+                const int jt = TABLE ? (t - lb) / nblk : 0;           // local tile index
+                const double* tb = TABLE ? s_beam + (jt % tch) * bpt * R::SIZE : s_beam;
                 if constexpr (TABLE) {
+                    if (chunked && jt % tch == 0) {       // a chunk starts: its records again
+                        if (jt > 0) __syncthreads();
+                        table_points(jt, r, rdr2, rdr);
+                        __syncthreads();
+                        table_dl(jt, dr, dr2);
+                        table_h(jt, 1, 2, M1, hm_rdr);
+                        table_h(jt, 2, 1, M3, hm_rdr2);
+                        __syncthreads();
+                    }
                     load_p0(t, cur);
                 } else {
                     if (MODE == 2) {
@@ -1050,15 +1075,28 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 #pragma unroll
             for (int v = 0; v < NMV + 1; ++v) acc[v] = 0.;
             double blk_err = 0.;
-            if constexpr (TABLE) {
-                table_h(3, 2, M4, hm_rdr);
-                if constexpr (COMP) table_hd();
+            if (TABLE && !chunked) {                  // the records of passes A and B stay
+                table_h(0, 3, 2, M4, hm_rdr);
+                if constexpr (COMP) table_hd(0);
                 __syncthreads();
             }
             for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
                 int nbt = STREAM ? tile_cells(t) : nbt_res;
-                const double* tb = TABLE ? s_beam + ((t - lb) / nblk) * bpt * R::SIZE : s_beam;
+                const int jt = TABLE ? (t - lb) / nblk : 0;           // local tile index
+                const double* tb = TABLE ? s_beam + (jt % tch) * bpt * R::SIZE : s_beam;
                 if constexpr (TABLE) {
+                    if (chunked && jt % tch == 0) {       // a chunk starts: its records again
+                        if (jt > 0) __syncthreads();
+                        table_points(jt, r, rdr2, rdr);
+                        __syncthreads();
+                        table_dl(jt, dr, dr2);
+                        table_h(jt, 0, 0, M0, hm_r);
+                        table_h(jt, 1, 2, M1, hm_rdr);
+                        table_h(jt, 2, 1, M3, hm_rdr2);
+                        table_h(jt, 3, 2, M4, hm_rdr);
+                        if constexpr (COMP) table_hd(jt);
+                        __syncthreads();
+                    }
                     load_p0(t, cur);
                 } else {
                     if (MODE == 2) {
