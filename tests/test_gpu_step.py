@@ -460,17 +460,22 @@ def test_run_ends_exactly_at_rn(module):
             assert max(np.abs(pe - oe).max(), np.abs(pb - ob).max()) <= PEE_TOL
 
 
+@pytest.mark.parametrize("shape", ["default", "mode4"])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_graph_replays_equal_eager_launches(prec):
+def test_graph_replays_equal_eager_launches(prec, shape):
     """xf_run_graph replays one instantiated graph of 4 launches x 25 steps;
     after every replay (with eager launches and synchronisations of another
     context in between) the state, radius and step equal those of the same
-    launches run eagerly, bit for bit."""
+    launches run eagerly, bit for bit (default shape, and streaming mode 4
+    with several tiles per workgroup)."""
     capi = _capi()
     pr = capi.XF_FP64 if prec == "fp64" else capi.XF_FP32
     g, p = _golden("ma_r50_ih.npz")
-    with capi.Context(p, precision=pr) as cg, capi.Context(p, precision=pr) as ce:
+    grid = 8 if shape == "mode4" else 0
+    with capi.Context(p, precision=pr, grid=grid) as cg, capi.Context(p, precision=pr, grid=grid) as ce:
         for c in (cg, ce):
+            if shape == "mode4":
+                c.set_launch(4, 2, 64 * (((2 * p.ebins + 1) // 2 + 63) // 64), 2)
             c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
         for rep in range(3):
             info = cg.run_graph(4, 25)
