@@ -309,16 +309,15 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     }
     XfLaunch Lq = L;
     Lq.bpt = max_bpt_used;              // LDS per workgroup depends on the largest tile
-    Lq.xrows = 1;                       // ... and on the exchange staging (rows of one instance)
     Lq.wbeams = max_bpt_used;           // ... and in mode 4 on the records of one workgroup's tiles
     int maxb = 0, rc = 0;
-    // the staging grows with the workgroup count and may lower the occupancy:
-    // settle both (the counts only shrink)
+    // in mode 4 the records grow as the workgroups get fewer and may lower
+    // the occupancy: settle both (the counts only shrink)
     for (int pass = 0; pass < 4; ++pass) {
         rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
         if (rc) return rc;
         grid = 0;
-        int xrows = 1, wbeams = max_bpt_used;
+        int wbeams = max_bpt_used;
         if (L.mode != 2 && L.mode != 4) {
             for (size_t i = 0; i < E->inst.size(); ++i) nblk[i] = tiles[i];
         } else {
@@ -342,28 +341,23 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
         }
         for (size_t i = 0; i < E->inst.size(); ++i) {
             grid += nblk[i];
-            xrows = std::max(xrows, nblk[i]);
             wbeams = std::max(wbeams, (tiles[i] + nblk[i] - 1) / nblk[i] * bpts[i]);
         }
         if (L.mode == 4) {
             // the records of one workgroup's tiles, as many as LDS holds (the
             // kernel takes its tiles in chunks of wbeams / bpt)
-            XfLaunch Lr = Lq;
-            Lr.xrows = std::max(Lq.xrows, xrows);
-            const int cap = xf_step_max_records(Lr, E->geom);
+            const int cap = xf_step_max_records(Lq, E->geom);
             if (cap < max_bpt_used)
                 return xf_fail(XF_EINVAL, "mode 4: LDS holds %d beam records, a tile has %d", cap, max_bpt_used);
             wbeams = std::min(wbeams, cap);
         }
-        if (xrows <= Lq.xrows && (L.mode != 4 || wbeams <= Lq.wbeams)) break;
-        Lq.xrows = std::max(Lq.xrows, xrows);
-        if (L.mode == 4) Lq.wbeams = std::max(Lq.wbeams, wbeams);
+        if (L.mode != 4 || wbeams <= Lq.wbeams) break;
+        Lq.wbeams = wbeams;
     }
     rc = xf_step_max_blocks_geom(Lq, E->geom, &maxb);
     if (rc) return rc;
     if (grid > maxb)
         return xf_fail(XF_EINVAL, "shape needs %d co-resident workgroups, device holds %d", grid, maxb);
-    L.xrows = Lq.xrows;
     L.wbeams = Lq.wbeams;
     if (!commit) {
         L.grid = grid;

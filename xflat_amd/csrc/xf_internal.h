@@ -117,7 +117,6 @@ struct XfLaunch {
     int block;         // threads per workgroup
     int bpt;           // beams per tile
     int grid;          // workgroups (all instances)
-    int xrows;         // largest workgroup count of one instance (exchange staging in LDS)
     int wbeams;        // mode 4: largest beam count of one workgroup's tiles (records in LDS)
 };
 

@@ -42,11 +42,10 @@ static int record_doubles(int geom) {
 }
 
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
-    int nm = geom == XF_MAA ? 4 : geom == XF_SA ? 1 : 2;
-    int bsize = record_doubles(geom);
-    // the pass-C error buffer doubles as the exchange staging: the lanes'
-    // values (in chunks of values) and the fetched rows [2 * 3 * nm][xrows]
-    size_t errb = std::max((size_t)4 * L.block * L.cpt, (size_t)6 * nm * std::max(L.xrows, 1));
+    const int bsize = record_doubles(geom);
+    // the pass-C error buffer doubles as the publish staging of the lanes'
+    // values (in chunks of values)
+    size_t errb = (size_t)4 * L.block * L.cpt;
     const size_t recs = L.mode == 4 ? (size_t)std::max(L.wbeams, L.bpt) : (size_t)L.bpt;   // beam records
     return sizeof(double) * (recs * bsize + errb + 4 * (size_t)L.bpt + XF_NVMAX);
 }

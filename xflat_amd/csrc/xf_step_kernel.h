@@ -52,9 +52,6 @@ namespace {
 
 constexpr double INV_2PI_D = 0.15915494309189535;
 constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;   // 2 s at 100 MHz
-#ifndef XF_EXCHANGE_WORDS
-#define XF_EXCHANGE_WORDS 16     // words per thread in flight in the exchange fetch
-#endif
 
 template <class T> struct Sp { T ar, ai, br, bi; };
 
@@ -592,7 +589,7 @@ __device__ __forceinline__ int rows_current(const unsigned long long* rows, int 
 }
 
 template <int NV, int NMAX>
-__device__ __forceinline__ bool xch_gather(double* s_tot, double* s_x, const XfInstDev& I, int lb, unsigned& gen,
+__device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, int lb, unsigned& gen,
                                            int* status, int* s_flag, long long* tr = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int B = blockDim.x;
@@ -684,7 +681,7 @@ template <int NV, int NMAX>
 __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot, double* s_x, int xcap, const XfInstDev& I,
                                                int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr) {
     xch_publish<NV, NMAX>(acc, s_x, xcap, I, lb, gen, tr);
-    return xch_gather<NV, NMAX>(s_tot, s_x, I, lb, gen, status, s_flag, tr);
+    return xch_gather<NV, NMAX>(s_tot, I, lb, gen, status, s_flag, tr);
 }
 }  // namespace
 
@@ -734,8 +731,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     const int tch = TABLE ? max(1, min(I.tchunk, ntl)) : 1;
     const bool chunked = TABLE && tch < ntl;                  // workgroup-uniform
     double* s_errb = s_beam + tch * bpt * R::SIZE;           // [4][B*CPT], between passes the
-                                                             // exchange's staging [2*NMV][nblk]
-    const int xcap = max(4 * B * CPT, 2 * NMV * nblk);     // doubles of s_errb
+                                                             // publish staging of the lanes' values
+    const int xcap = 4 * B * CPT;                           // doubles of s_errb
     double* s_errs = s_errb + xcap;                          // [bpt][4]
     double* s_tot = s_errs + 4 * bpt;                        // [XF_NVMAX]
     __shared__ int s_flag[2];
