@@ -513,8 +513,8 @@ def main():
     ap.add_argument("--launch", default=None, help="mode,cpt,bpt: fixed launch shape instead of autotuning")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported in this build (the angle-sharded "
-              "multi-GPU design is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (several GPUs would shard the "
+              "angle bins, one all-reduce of the moments per exchange; that path is not built).", file=sys.stderr)
         sys.exit(2)
     w = WORKLOADS[a.workload]
     a.steps = a.steps if a.steps is not None else w["steps"]
