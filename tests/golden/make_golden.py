@@ -43,11 +43,11 @@ REF_DIR = os.path.join(ROOT, "oracle", "_ref")
 
 # name -> (config overrides, seed or None for pure states, amplitude, records to keep)
 CASES = {
-    # configs[0] physics (bulb MA, inverted hierarchy) at R_nu, small grid:
+    # the bulb MA benchmark physics (inverted hierarchy) at R_nu, small grid:
     # the first attempts with dr = 1e-3 km are rejected until dr ~ 4e-8 km.
     "ma_r10_ih": (dict(Abins=32, Ebins=16, R0=10, Ts=400, dm2="-2.4E-3"), 11, 1e-6, 6),
     "ma_r10_nh": (dict(Abins=32, Ebins=16, R0=10, Ts=300, dm2="2.4E-3"), 12, 1e-6, 6),
-    # further out: larger steps, tighter tolerance (configs[1] uses eps0 = 1e-9)
+    # further out: larger steps, tighter tolerance (the 1000 x 400 benchmark uses eps0 = 1e-9)
     "ma_r50_ih": (dict(Abins=48, Ebins=24, R0=50, Ts=300, dm2="-2.4E-3", eps0="1.E-9"), 13, 1e-6, 6),
     # no matter term (hasMatter= 0), vacuum + nu-nu only
     "ma_vac": (dict(Abins=16, Ebins=16, R0=30, Ts=200, hasMatter=0), 14, 1e-6, 6),

@@ -1,6 +1,6 @@
 """Parity of the HIP evolution step (through the C-ABI) with the reference.
 
-FP64 bar (north-star): max |dP_ee| <= 1e-9 and max |dPbar_ee| <= 1e-9 per beam
+FP64 bar: max |dP_ee| <= 1e-9 and max |dPbar_ee| <= 1e-9 per beam
 and energy, with the same accepted-step sequence.  Mixed precision is
 compared with the FP64 GPU run.
 """
@@ -191,7 +191,7 @@ def test_phy_mirror_one_step_matches_oracle(name):
 
 
 def test_fused_vs_oracle_bulb_200x100():
-    """configs[0] grid (MA 200 angles x 100 energies) against the oracle."""
+    """The 200 x 100 bulb benchmark grid (MA 200 angles x 100 energies) against the oracle."""
     capi = _capi()
     from oracle.oracle import Oracle
     from xflat_amd.config import bulb_params

@@ -174,7 +174,7 @@ def parse_config_text(text: str, module: int | None = None) -> XfParams:
     return p
 
 
-# Physics of the bulb benchmark (configs of the north-star metric): flavours
+# Physics of the bulb benchmark (every benchmark configuration): flavours
 # with 1e51 erg/s each, Fermi-Dirac spectra with zero chemical potential at
 # T = 2.76, 4.01, 6.26 MeV, delta m^2 = 2.4e-3 eV^2, theta_v = 0.1, and an
 # electron density n_e = 1e32 cm^-3 (10 km / r)^3: nb0 = 0, Ye = 0.5,
