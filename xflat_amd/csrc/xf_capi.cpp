@@ -401,9 +401,11 @@ static int choose_shape(xf_ensemble* E) {
     int ne_max = 1;
     for (auto& in : E->inst) ne_max = std::max(ne_max, in->t.ne);
     std::vector<int> modes, cpts;
-    if (o.resident == 0) modes = {2};
+    // streaming: mode 4 (mode 2's results, fewer set-ups), else mode 2 where
+    // a tile's beam records do not fit LDS
+    if (o.resident == 0) modes = {4, 2};
     else if (o.resident == 1) modes = {0, 1};
-    else modes = {0, 1, 2};
+    else modes = {0, 1, 4, 2};
     if (o.cpt > 0) cpts = {o.cpt};
     else cpts = {1, 2, 4};
     for (int mode : modes) {

@@ -1,7 +1,7 @@
 // xflat_run -- command-line host for the MI355X evolution step.
 //
 //   xflat_run <config.txt> [snapshot.nc] [--precision fp64|mixed|fp32]
-//             [--module ma|maa|sa] [--chunk N] [--species 2|4] [--quiet]
+//             [--module ma|maa|sa] [--chunk N] [--species 2|4] [--autotune N] [--quiet]
 //
 // --module stands for the reference's compile-time module choice (-DMA,
 // -DMAA, -DSA, include/global.h:5-19); by default MAA when Pbins > 1, else MA.
@@ -77,7 +77,7 @@ struct Stream {
 int main(int argc, char** argv) {
     const char* cfg = nullptr;
     const char* init = nullptr;
-    int precision = XF_FP64, species = 0, chunk = 2000, module = -1;
+    int precision = XF_FP64, species = 0, chunk = 2000, module = -1, autotune = 0;
     bool quiet = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -95,6 +95,8 @@ int main(int argc, char** argv) {
             chunk = std::max(1, std::atoi(argv[++i]));
         } else if (a == "--species" && i + 1 < argc) {
             species = std::atoi(argv[++i]);
+        } else if (a == "--autotune" && i + 1 < argc) {
+            autotune = std::max(0, std::atoi(argv[++i]));
         } else if (a == "--quiet") {
             quiet = true;
         } else if (!cfg) {
@@ -108,7 +110,7 @@ int main(int argc, char** argv) {
     }
     if (!cfg) {
         std::fprintf(stderr, "usage: xflat_run <config.txt> [snapshot.nc] [--precision fp64|mixed|fp32] "
-                             "[--module ma|maa|sa] [--chunk N] [--species 2|4] [--quiet]\n");
+                             "[--module ma|maa|sa] [--chunk N] [--species 2|4] [--autotune N] [--quiet]\n");
         return 2;
     }
 
@@ -136,6 +138,16 @@ int main(int argc, char** argv) {
         CHECK(xf_state_upload(ctx, psi.data(), r, dr));
     } else {
         CHECK(xf_state_init_pure(ctx));
+    }
+    if (autotune > 0) {
+        // time every launch shape over `autotune` steps from this state (which
+        // the autotuner restores) and keep the fastest
+        xf_opts best;
+        double ms = 0.;
+        CHECK(xf_autotune(ctx, autotune, &best, &ms));
+        if (!quiet)
+            std::printf("Launch shape: mode %d, %d cells per thread, %d threads (%.4f ms per step)\n", best.resident,
+                        best.cpt, best.block, ms);
     }
 
     std::vector<double> fv(4 * (size_t)p.ebins), hvac(2 * (size_t)p.ebins), le(4), ev(4);
