@@ -404,7 +404,9 @@ def bench_ma(a, prec_name):
             print(f"[bench] normal hierarchy: {prec_name} {out['normal_hierarchy']['value']:.3e}, fp64 "
                   f"{out['normal_hierarchy']['fp64_value']:.3e} beam-steps/s", file=sys.stderr, flush=True)
         if not a.no_extra:
-            out["short_launches"] = time_short_launches(p, prec_name, psi0, launch)
+            li = res["launch"]                        # the headline's (tuned) shape
+            out["short_launches"] = time_short_launches(p, prec_name, psi0,
+                                                        (li["mode"], li["cpt"], li["beams_per_tile"]))
             s = out["short_launches"]
             print(f"[bench] {s['launches']} launches x {s['steps_per_launch']} steps: eager {s['eager_value']:.3e}, "
                   f"graph {s['graph_value']:.3e} beam-steps/s", file=sys.stderr, flush=True)
