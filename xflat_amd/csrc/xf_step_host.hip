@@ -47,7 +47,7 @@ size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
     // values (in chunks of values)
     size_t errb = (size_t)4 * L.block * L.cpt;
     const size_t recs = L.mode == 4 ? (size_t)std::max(L.wbeams, L.bpt) : (size_t)L.bpt;   // beam records
-    return sizeof(double) * (recs * bsize + errb + 4 * (size_t)L.bpt + XF_NVMAX);
+    return sizeof(double) * (recs * bsize + errb + 4 * (size_t)L.bpt);
 }
 
 // Mode 4: the most beam records that fit the workgroup's LDS beside the rest

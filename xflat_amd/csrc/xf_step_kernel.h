@@ -734,10 +734,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                                              // publish staging of the lanes' values
     const int xcap = 4 * B * CPT;                           // doubles of s_errb
     double* s_errs = s_errb + xcap;                          // [bpt][4]
-    double* s_tot = s_errs + 4 * bpt;                        // [XF_NVMAX]
     __shared__ int s_flag[2];
-    // angular moments, uniform over the workgroup: psi0, psi1, psi3, psi4, candidate
-    __shared__ double s_M0[12], s_M1[12], s_M3[12], s_M4[12], s_Mn[12];
+    // angular moments, uniform over the workgroup, gathered in place by the
+    // exchanges: psi1 and psi3 (pass A), psi4 (pass B), and two buffers that
+    // take turns as psi0 and the candidate (pass C, with the step error
+    // after the moments)
+    __shared__ double s_MA[2 * NMV], s_MB[NMV], s_MC[2][NMV + 1];
 
     // uniform step state
     double r = rs->r, dr = rs->dr;
@@ -749,11 +751,11 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     int* status = &rs->status;
 
     Cells<ST, CT, NS, CPT> C;
-    double* M0 = s_M0;
-    double* M1 = s_M1;
-    double* M3 = s_M3;
-    double* M4 = s_M4;
-    double* Mn = s_Mn;
+    double* M0 = s_MC[0];
+    const double* M1 = s_MA;
+    const double* M3 = s_MA + NMV;
+    const double* M4 = s_MB;
+    double* Mn = s_MC[1];
 
     // ---- tile helpers -------------------------------------------------------
     auto tile_cells = [&](int t) __attribute__((always_inline)) {
@@ -857,8 +859,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         double dr2 = .5 * dr, rdr = r + dr, rdr2 = r + dr2;
         for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
             int nbt = STREAM ? tile_cells(t) : nbt_res;
-                const int jt = TABLE ? (t - lb) / nblk : 0;           // local tile index
-                const double* tb = TABLE ? s_beam + (jt % tch) * bpt * R::SIZE : s_beam;
+            const int jt = TABLE ? (t - lb) / nblk : 0;           // local tile index
+            const double* tb = TABLE ? s_beam + (jt % tch) * bpt * R::SIZE : s_beam;
             if constexpr (TABLE) {
                 if (jt % tch == 0) {                  // a chunk starts: its records
                     if (jt > 0) __syncthreads();
@@ -885,16 +887,16 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 acc_moments<NM>(acc, rec + R::W + 0 * NM, (double)X0, (double)X2, (double)X3);
             }
         }
-        ok = reduce_publish<NMV, 0>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag);
-        if (tid < NMV) M0[tid] = s_tot[tid];
-        __syncthreads();
+        ok = reduce_publish<NMV, 0>(acc, M0, s_errb, xcap, I, lb, gen, status, s_flag);
     }
 
     long long steps_call = 0, acc_call = 0;
     int stop_reason = 0;
     double hm_cur = hm_at(I, r);
+    // Every exchange ends with a workgroup barrier after its last LDS access,
+    // and every reader of the beam records, the staging and the moments comes
+    // before one: a step starts without a barrier of its own.
     while (ok) {
-        __syncthreads();
         if (!(steps_call < max_steps)) { stop_reason = 0; break; }
         if (!(r < I.Rn)) { stop_reason = 1; break; }
         const double dr2 = .5 * dr;
@@ -978,9 +980,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
                 }
             }
-            ok = reduce_publish<2 * NMV, 0>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
-            if (tid < NMV) { M1[tid] = s_tot[tid]; M3[tid] = s_tot[NMV + tid]; }
-            __syncthreads();
+            ok = reduce_publish<2 * NMV, 0>(acc, s_MA, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
             if (!ok) break;
         }
 
@@ -1059,9 +1059,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
                 }
             }
-            ok = reduce_publish<NMV, 0>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
-            if (tid < NMV) M4[tid] = s_tot[tid];
-            __syncthreads();
+            ok = reduce_publish<NMV, 0>(acc, s_MB, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
             if (!ok) break;
         }
 
@@ -1261,10 +1259,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (tr && tid == 0) tr[15] = (long long)__builtin_amdgcn_s_memrealtime();
             }
             acc[NMV] = blk_err;
-            ok = reduce_publish<NMV, 1>(acc, s_tot, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
-            if (tid < NMV) Mn[tid] = s_tot[tid];
-            errmax = s_tot[NMV];
-            __syncthreads();
+            ok = reduce_publish<NMV, 1>(acc, Mn, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
+            errmax = Mn[NMV];
             if (!ok) break;
         }
 
@@ -1296,7 +1292,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         }
                     }
             }
-            if (tid < NMV) M0[tid] = Mn[tid];
+            double* m = M0;                      // the candidate's moments become psi0's
+            M0 = Mn;
+            Mn = m;
             ++accepted;
             ++acc_call;
         }
