@@ -68,9 +68,8 @@ __device__ __forceinline__ double lam_d(double h0, double h1, double h2) {
 // (-1)^k x^2k/(2k+1)!, truncated after x^16: remainder < 1e-23): FMAs only, no
 // square root, division or transcendental call, which on CDNA4 (FP64 FMA at
 // the FP32 rate) is the cheap path.  Larger angles use sqrt/sincos/divide.
-__device__ __forceinline__ void prop_d(double h0, double h1, double h2, double lam2, double dl, PropD& P) {
+__device__ __forceinline__ void prop_ds(double lam2, double dl, double& c, double& sc) {
     const double u = lam2 * (dl * dl);
-    double c, sc;
     if (u < 0.0625) {
         double sn = 1.0 / 355687428096000.0;               // 1/17!
         sn = fma(u, -sn, 1.0 / 1307674368000.0);           // 1/15!
@@ -99,10 +98,29 @@ __device__ __forceinline__ void prop_d(double h0, double h1, double h2, double l
         sincos(ldr, &s, &c);
         sc = s / lam;
     }
+}
+__device__ __forceinline__ PropD pd_of(double h0, double h1, double h2, double c, double sc) {
+    PropD P;
     P.m0 = c;
     P.m1 = -h0 * sc;
     P.m2 = h2 * sc;
     P.m3 = -h1 * sc;
+    return P;
+}
+__device__ __forceinline__ void prop_d(double h0, double h1, double h2, double lam2, double dl, PropD& P) {
+    double c, sc;
+    prop_ds(lam2, dl, c, sc);
+    P = pd_of(h0, h1, h2, c, sc);
+}
+// U over twice the path length from U over dl (the double-angle identities
+// cos 2x = 2 cos^2 x - 1, sin(2x)/lambda = 2 cos x sin(x)/lambda), for the
+// sub-steps dr / c0 = 2 (dr / 2) / c0 of the multi-angle and single-angle
+// geometries
+__device__ __forceinline__ void prop_d2(double h0, double h1, double h2, double lam2, double dl, PropD& P2, PropD& P) {
+    double c, sc;
+    prop_ds(lam2, dl, c, sc);
+    P = pd_of(h0, h1, h2, c, sc);
+    P2 = pd_of(h0, h1, h2, fma(2.0 * c, c, -1.0), 2.0 * (sc * c));
 }
 __device__ __forceinline__ Sp<double> apply_d(const PropD& P, const Sp<double>& a) {
     Sp<double> o;
@@ -125,10 +143,9 @@ struct PropF { float cm1, m1, m2, m3; };
 // each is rounded relative to its own size, the remainders to x^2 rather than
 // to x, and no square root or transcendental is needed.  Larger angles take
 // the hardware sin/cos of the half angle.
-__device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, PropF& P, PropF& PR) {
+__device__ __forceinline__ void prop_f2s(float h0, float h1, float h2, float dl, float& cm1, float& sc, float& scr) {
     const float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
     const float x2 = lam2 * (dl * dl);
-    float cm1, sc, scr;
     if (x2 < 1.f) {
         // (cos x - 1)/t and (sin x / x - 1)/t as cubics in t = x^2: the
         // degree-3 Chebyshev interpolants on t in [0, 1] of their Taylor
@@ -154,14 +171,29 @@ __device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, 
         sc = 2.f * sh * ch * rl;
         scr = sc - dl;
     }
-    P.cm1 = cm1;
-    P.m1 = -h0 * sc;
-    P.m2 = h2 * sc;
-    P.m3 = -h1 * sc;
-    PR.cm1 = cm1;
-    PR.m1 = -h0 * scr;
-    PR.m2 = h2 * scr;
-    PR.m3 = -h1 * scr;
+}
+// the coefficient set (cm1, -i c H) of U - I or of a remainder
+__device__ __forceinline__ PropF pf_of(float h0, float h1, float h2, float cm1, float c) {
+    return PropF{cm1, -h0 * c, h2 * c, -h1 * c};
+}
+__device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, PropF& P, PropF& PR) {
+    float cm1, sc, scr;
+    prop_f2s(h0, h1, h2, dl, cm1, sc, scr);
+    P = pf_of(h0, h1, h2, cm1, sc);
+    PR = pf_of(h0, h1, h2, cm1, scr);
+}
+// Double angle.  Where one Hamiltonian acts over dl and over 2 dl (the first
+// two sub-steps of a step of the multi-angle and single-angle geometries,
+// dr / c0 and (dr / 2) / c0, which differ by exactly a factor of 2),
+// U(2 dl) = U(dl)^2, so with D = U(dl) - I
+//   U(2 dl) - I = 2 D + D^2,
+//   cos 2x - 1 = 2 cm1 (cm1 + 2),  sin(2x)/lambda = 2 sc (1 + cm1),
+// and the difference of the remainders R(dl) - R(2 dl) has the coefficients
+//   cm1 - (cos 2x - 1) = -cm1 (2 cm1 + 3),
+//   scr - (sin(2x)/lambda - 2 dl) = -(scr + 2 sc cm1),
+// both second order in x with terms of one sign (no cancellation).
+__device__ __forceinline__ PropF pf_rdiff2(float h0, float h1, float h2, float cm1, float sc, float scr) {
+    return pf_of(h0, h1, h2, -cm1 * fmaf(2.f, cm1, 3.f), -fmaf(2.f * sc, cm1, scr));
 }
 __device__ __forceinline__ Sp<float> delta_f(const PropF& P, const Sp<float>& a) {
     Sp<float> o;
@@ -700,6 +732,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     // tiles of passes A and B)
     constexpr bool STREAM = (MODE == 2 || MODE == 4);
     constexpr bool TABLE = (MODE == 4);
+    // multi-angle and single-angle: the first two sub-steps' path lengths
+    // differ by exactly a factor of 2 (U over the longer one by double angle)
+    constexpr bool DBL = (GEOM != XF_MAA);
     using R = BRec<GEOM>;
     constexpr int NM = R::NM;
     constexpr int NMV = NM * 3;
@@ -952,13 +987,34 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         if constexpr (F64) {
                             double lam = lam_d(h0, h1, h2);
                             PropD P1, P0;
-                            prop_d(h0, h1, h2, lam, rec[R::DL + 1], P1);
-                            prop_d(h0, h1, h2, lam, rec[R::DL + 0], P0);
+                            if constexpr (DBL) {
+                                prop_d2(h0, h1, h2, lam, rec[R::DL + 0], P1, P0);
+                            } else {
+                                prop_d(h0, h1, h2, lam, rec[R::DL + 1], P1);
+                                prop_d(h0, h1, h2, lam, rec[R::DL + 0], P0);
+                            }
                             Sp<double> a = sp_d(C.p0[k][s]);
                             Sp<double> v1 = apply_d(P1, a), v3 = apply_d(P0, a);
                             if (MODE == 0) { C.k1[k][s] = v1; C.k3[k][s] = v3; }
                             src_add<CT>(I, e, s, v1, X0, X2, X3);
                             src_add<CT>(I, e, s, v3, Y0, Y2, Y3);
+                        } else if constexpr (DBL) {
+                            // dl1 = 2 dl0: U(dl1) - I = 2 D0 + D0^2
+                            const float f0 = (float)h0, f1 = (float)h1, f2 = (float)h2;
+                            float cm1, sc, scr;
+                            prop_f2s(f0, f1, f2, (float)rec[R::DL + 0], cm1, sc, scr);
+                            const PropF P0 = pf_of(f0, f1, f2, cm1, sc);
+                            Sp<float> a = sp_f(C.p0[k][s]);
+                            Sp<float> d3 = delta_f(P0, a);
+                            Sp<float> d1 = sp_add(sp_add(d3, d3), delta_f(P0, d3));
+                            if (MODE == 0) {
+                                // kept for pass C: the remainder coefficients Q = R0 - R1
+                                // so far, and delta3
+                                C.k1[k][s] = pf_pack(pf_rdiff2(f0, f1, f2, cm1, sc, scr));
+                                C.k3[k][s] = d3;
+                            }
+                            src_add<CT>(I, e, s, sp_add(a, d1), X0, X2, X3);
+                            src_add<CT>(I, e, s, sp_add(a, d3), Y0, Y2, Y3);
                         } else {
                             PropF P1, P0, PR1, PR0;
                             prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], P1, PR1);
@@ -1136,8 +1192,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);
                                 double lam = lam_d(x0, x1, x2);
                                 PropD P1, P0, Pb, P4;
-                                prop_d(x0, x1, x2, lam, rec[R::DL + 1], P1);
-                                prop_d(x0, x1, x2, lam, rec[R::DL + 0], P0);
+                                if constexpr (DBL) {
+                                    prop_d2(x0, x1, x2, lam, rec[R::DL + 0], P1, P0);
+                                } else {
+                                    prop_d(x0, x1, x2, lam, rec[R::DL + 1], P1);
+                                    prop_d(x0, x1, x2, lam, rec[R::DL + 0], P0);
+                                }
                                 q1 = apply_d(P1, a);
                                 q3 = apply_d(P0, a);
                                 cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);
@@ -1178,11 +1238,19 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 } else {
                                 double x0, x1, x2;
                                 cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);                    // h0
+                                if constexpr (DBL) {
+                                    const float f0 = (float)x0, f1 = (float)x1, f2 = (float)x2;
+                                    float cm1, sc, scr;
+                                    prop_f2s(f0, f1, f2, (float)rec[R::DL + 0], cm1, sc, scr);
+                                    d3 = delta_f(pf_of(f0, f1, f2, cm1, sc), a);
+                                    Q = pf_rdiff2(f0, f1, f2, cm1, sc, scr);
+                                } else {
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
                                 Q = PropF{0.f - PR.cm1, 0.f - PR.m1, 0.f - PR.m2, 0.f - PR.m3};
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 0], P, PR);
                                 d3 = delta_f(P, a);
                                 Q = pf_add(Q, PR);
+                                }
                                 cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);                    // h1
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
                                 Q = pf_sub(Q, PR);
