@@ -70,7 +70,21 @@ __device__ __forceinline__ double lam_d(double h0, double h1, double h2) {
 // the FP32 rate) is the cheap path.  Larger angles use sqrt/sincos/divide.
 __device__ __forceinline__ void prop_ds(double lam2, double dl, double& c, double& sc) {
     const double u = lam2 * (dl * dl);
-    if (u < 0.0625) {
+    if (u < 0x1p-10) {
+        // small angles (the common case at small dr): the same series
+        // truncated after x^8 / x^9; the first omitted terms, u^5/10! on
+        // cos x and u^5/11! on sin(x)/x, are below 2.5e-22
+        double sn = 1.0 / 362880.0;                        // 1/9!
+        sn = fma(u, -sn, 1.0 / 5040.0);
+        sn = fma(u, -sn, 1.0 / 120.0);
+        sn = fma(u, -sn, 1.0 / 6.0);
+        double cs = 1.0 / 40320.0;                         // 1/8!
+        cs = fma(u, -cs, 1.0 / 720.0);
+        cs = fma(u, -cs, 1.0 / 24.0);
+        cs = fma(u, -cs, 0.5);
+        c = fma(-u, cs, 1.0);
+        sc = dl * fma(-u, sn, 1.0);
+    } else if (u < 0.0625) {
         double sn = 1.0 / 355687428096000.0;               // 1/17!
         sn = fma(u, -sn, 1.0 / 1307674368000.0);           // 1/15!
         sn = fma(u, sn, -1.0 / 6227020800.0);              // 1/13!
