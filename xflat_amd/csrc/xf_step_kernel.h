@@ -160,7 +160,17 @@ struct PropF { float cm1, m1, m2, m3; };
 __device__ __forceinline__ void prop_f2s(float h0, float h1, float h2, float dl, float& cm1, float& sc, float& scr) {
     const float lam2 = h0 * h0 + (h1 * h1 + h2 * h2);
     const float x2 = lam2 * (dl * dl);
-    if (x2 < 1.f) {
+    if (__all(x2 < 0x1p-10f)) {
+        // every lane of the wave at a small angle (a wave-uniform branch):
+        // the same functions to first order in t, whose omitted t^2/720 and
+        // t^2/5040 are below 2.7e-9 and 1.2e-9 relative for t < 2^-10
+        const float pc = fmaf(x2, 0.0416666679f, -0.5f);
+        const float ps = fmaf(x2, 0.00833333377f, -0.166666672f);
+        cm1 = x2 * pc;
+        const float S = x2 * ps;
+        scr = dl * S;
+        sc = dl + scr;
+    } else if (x2 < 1.f) {
         // (cos x - 1)/t and (sin x / x - 1)/t as cubics in t = x^2: the
         // degree-3 Chebyshev interpolants on t in [0, 1] of their Taylor
         // series (Chebyshev economization), coefficients rounded to FP32.
