@@ -990,7 +990,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     }
                     load_p0(t, cur);
                 } else {
-                    __syncthreads();                      // the previous tile is done with s_beam
+                    // the previous tile is done with s_beam (resident modes: the
+                    // last exchange ended with a barrier after every read of it)
+                    if (MODE == 2) __syncthreads();
                     if (MODE == 2) load_p0(t, cur);      // in flight during the set-up
                     tile_points(t, nbt, r, rdr2, rdr);
                     __syncthreads();
@@ -1094,9 +1096,9 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         __syncthreads();
                         load_p0(t, cur);                  // in flight during the set-up
                         tile_points(t, nbt, r, rdr2, rdr);
+                        __syncthreads();
+                        tile_dl(nbt, dr, dr2);
                     }
-                    __syncthreads();
-                    if (MODE == 2) tile_dl(nbt, dr, dr2);
                     tile_h(nbt, 1, 2, M1, hm_rdr);
                     tile_h(nbt, 2, 1, M3, hm_rdr2);
                     __syncthreads();
@@ -1183,8 +1185,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         tile_h(nbt, 0, 0, M0, hm_r);
                         tile_h(nbt, 1, 2, M1, hm_rdr);
                         tile_h(nbt, 2, 1, M3, hm_rdr2);
+                        __syncthreads();
                     }
-                    __syncthreads();
                     tile_h(nbt, 3, 2, M4, hm_rdr);
                     if constexpr (COMP)
                         for (int b = tid; b < nbt; b += B) beam_hd<GEOM>(s_beam + b * R::SIZE);
