@@ -200,6 +200,26 @@ int  xf_state_species(const xf_ctx* ctx);        /* 2 or 4 in use           */
  * condition fires).  log (optional) receives one record per attempt.       */
 int  xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop,
                   xf_step_rec* log, xf_run_info* info);
+/* xf_run_steps without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The same
+ * kernels, steps and results as xf_run_steps on the same context (bit for
+ * bit); the launch descriptor, the exchange rows and the step log live in
+ * one static device array of XF_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends, instead of the context's own buffers.  The context's
+ * state buffers are the inputs and outputs, as for xf_run_steps.
+ * Preconditions: a launch shape chosen at set-up (xf_get_launch,
+ * xf_set_launch or xf_autotune, which may allocate), at most
+ * XF_NOALLOC_MAX_WORKGROUPS workgroups, and with a log at most
+ * XF_NOALLOC_MAX_LOG attempts; otherwise XF_EINVAL.  Every call on a GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (two host
+ * threads, or two streams) race and give undefined results.  Each call
+ * clears what must start at zero (status, rows) before its stepper runs. */
+#define XF_NOALLOC_MAX_WORKGROUPS 1024
+#define XF_NOALLOC_MAX_LOG        4096
+#define XF_NOALLOC_SCRATCH_BYTES  4330496   /* 1 KiB descriptor + 4 KiB map + 128 KiB log + 4 MiB rows */
+int  xf_run_steps_noalloc(xf_ctx* ctx, int max_steps, const xf_stop* stop,
+                          xf_step_rec* log, xf_run_info* info);
 /* The radial loop as a HIP graph: `launches` launches of up to max_steps
  * attempted steps each, replayed from one instantiated graph (built on first
  * use, kept while launches, max_steps and the launch shape stay).  The same

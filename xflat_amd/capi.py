@@ -93,6 +93,7 @@ SIGNATURES = {
     "xf_state_counters": (_I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "xf_state_species": (_I, [_P]),
     "xf_run_steps": (_I, [_P, _I, ctypes.POINTER(XfStop), _P, ctypes.POINTER(XfRunInfo)]),
+    "xf_run_steps_noalloc": (_I, [_P, _I, ctypes.POINTER(XfStop), _P, ctypes.POINTER(XfRunInfo)]),
     "xf_run_graph": (_I, [_P, _I, _I, ctypes.POINTER(XfRunInfo)]),
     "xf_get_launch": (_I, [_P, _OP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "xf_set_launch": (_I, [_P, _I, _I, _I, _I]),
@@ -233,6 +234,17 @@ class Context:
         info = XfRunInfo()
         check(self.L.xf_run_steps(self.h, max_steps, ctypes.byref(stop) if stop else None,
                                   rec.ctypes.data if log else None, ctypes.byref(info)))
+        if log:
+            rec = rec[:info.steps]
+        return info, rec
+
+    def run_noalloc(self, max_steps: int, log: bool = False, stop: XfStop | None = None):
+        """run() through xf_run_steps_noalloc (no device allocation; the
+        launch shape must have been chosen, e.g. by launch())."""
+        rec = np.zeros(max(max_steps, 1), dtype=STEP_DTYPE) if log else None
+        info = XfRunInfo()
+        check(self.L.xf_run_steps_noalloc(self.h, max_steps, ctypes.byref(stop) if stop else None,
+                                          rec.ctypes.data if log else None, ctypes.byref(info)))
         if log:
             rec = rec[:info.steps]
         return info, rec

@@ -824,6 +824,53 @@ int xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop, xf_step_rec* l
     return fill_info(in, info, ms);
 }
 
+// xf_run_steps with the descriptor, the rows and the log in the static
+// scratch array of xf_noalloc.hip: no allocation on this path (the launch
+// shape, the stream and the events were made at set-up).
+int xf_run_steps_noalloc(xf_ctx* ctx, int max_steps, const xf_stop* stop, xf_step_rec* log, xf_run_info* info) {
+    if (!ctx || max_steps < 0) return xf_fail(XF_EINVAL, "xf_run_steps_noalloc: bad arguments");
+    xf_ensemble* E = ctx->ens;
+    if (E->inst.size() != 1) return xf_fail(XF_EINVAL, "xf_run_steps_noalloc: one instance per context");
+    XfInst* in = E->inst[0].get();
+    if (!E->shaped)
+        return xf_fail(XF_EINVAL, "xf_run_steps_noalloc: no launch shape; choose one at set-up "
+                                  "(xf_get_launch, xf_set_launch or xf_autotune)");
+    if (in->dev.nblk > XF_NOALLOC_MAX_WORKGROUPS || E->L.grid > XF_NOALLOC_MAX_WORKGROUPS)
+        return xf_fail(XF_EINVAL, "xf_run_steps_noalloc: the launch has %d workgroups, the scratch array holds "
+                                  "exchange rows for %d", E->L.grid, XF_NOALLOC_MAX_WORKGROUPS);
+    if (log && max_steps > XF_NOALLOC_MAX_LOG)
+        return xf_fail(XF_EINVAL, "xf_run_steps_noalloc: a log of %d attempts exceeds the scratch array's %d",
+                       max_steps, XF_NOALLOC_MAX_LOG);
+    if (3ll * max_steps + 1 > XF_MAX_EXCHANGES)
+        return xf_fail(XF_EINVAL, "%d steps in one launch exceed the exchange tags", max_steps);
+    XfNoAllocArea A{};
+    int rc = xf_noalloc_area(&A);
+    if (rc) return rc;
+    XfInstDev d = in->dev;
+    d.rows = A.rows;
+    d.log = log ? A.log : nullptr;
+    d.trace = nullptr;
+    d.trace_steps = 0;
+    d.dump_rstep = stop ? stop->dump_rstep : 0.;
+    d.dump_last_r = stop ? stop->dump_last_r : 0.;
+    d.dump_itrstep = stop ? stop->dump_itrstep : 0;
+    if ((rc = xf_noalloc_setup(d, A, E->L.grid, E->stream))) return rc;
+    if ((rc = xf_step_prepare(A.desc, 1, E->stream))) return rc;
+    HIPCHK(hipEventRecord(E->ev0, E->stream));
+    if ((rc = xf_step_launch_geom(E->L, E->geom, A.desc, A.blk, max_steps, E->stream))) return rc;
+    HIPCHK(hipEventRecord(E->ev1, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, E->ev0, E->ev1));
+    if (log) {
+        XfRunScalars rs;
+        if ((rc = in->scalars(&rs))) return rc;
+        if (rs.steps_call > 0)
+            HIPCHK(hipMemcpy(log, A.log, (size_t)rs.steps_call * sizeof(xf_step_rec), hipMemcpyDeviceToHost));
+    }
+    return fill_info(in, info, f);
+}
+
 int xf_run_graph(xf_ctx* ctx, int launches, int max_steps, xf_run_info* info) {
     if (!ctx) return xf_fail(XF_EINVAL, "xf_run_graph: null ctx");
     xf_ensemble* E = ctx->ens;

@@ -131,6 +131,18 @@ int xf_step_func_info(const XfLaunch& L, int geom, int* vgprs, int* spill_bytes)
 // Clears the run status and the exchange rows of every instance of a launch
 // (a kernel, so the launch sequence holds no memset node).
 int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream);
+// The no-allocation entry point's device memory (xf_noalloc.hip): the
+// regions of the static scratch array on the current device.
+struct XfNoAllocArea {
+    XfInstDev* desc;               // launch descriptor of the one instance
+    int* blk;                      // workgroup -> instance map (all 0)
+    xf_step_rec* log;              // per-attempt log
+    unsigned long long* rows;      // exchange rows for up to XF_NOALLOC_MAX_WORKGROUPS
+};
+int xf_noalloc_area(XfNoAllocArea* out);
+// Writes descriptor `d` into the area and clears the map for `grid`
+// workgroups (a kernel on `stream`: no copy or memset node).
+int xf_noalloc_setup(const XfInstDev& d, const XfNoAllocArea& a, int grid, void* stream);
 // The radial loop as an instantiated graph of `launches` launches of up to
 // max_steps steps (the first without clearing: the caller clears before it).
 int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int ninst,
