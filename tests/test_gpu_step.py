@@ -533,7 +533,11 @@ def test_streaming_table_mode_equals_streaming(prec):
              ("sa", dict(module=2, Abins=1, Ebins=200, R0=12), (2, 1), (0, 8)),
              # 16 tiles of 64 beams per workgroup: more records than LDS
              # holds, so mode 4 takes them in chunks
-             ("ma chunked", dict(Abins=4000, Ebins=8, R0=10), (1, 64), (4,))]
+             ("ma chunked", dict(Abins=4000, Ebins=8, R0=10), (1, 64), (4,)),
+             # the bench grid in one-beam tiles of 256 threads with every
+             # co-resident workgroup: the kernel for this block holds its
+             # state in up to 256 VGPRs with no scratch
+             ("ma 1000x400", dict(Abins=1000, Ebins=400, R0=10, eps0="1.E-9"), (2, 1), (0,))]
     for name, kw, (cpt, bpt), grids in cases:
         p = bulb_params(**kw)
         for grid in grids:

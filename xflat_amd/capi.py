@@ -5,8 +5,11 @@ there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -124,6 +127,23 @@ SIGNATURES = {
 }
 
 
+# Handles still open: closed by an exit handler, before the interpreter
+# finalises its modules and before the HIP runtime's own exit handlers run,
+# so that no device memory is freed from a finaliser or a static destructor.
+_LIVE = weakref.WeakSet()
+
+
+def _close_all():
+    for obj in list(_LIVE):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_all)
+
+
 def load():
     """Loads libxflat.so (raises if it has not been built)."""
     global _LIB
@@ -185,6 +205,7 @@ class Context:
         h = ctypes.c_void_p()
         check(self.L.xf_create(ctypes.byref(params), ctypes.byref(self.opts), ctypes.byref(h)))
         self.h = h
+        _LIVE.add(self)
         self.nbeam = n_beams(params)
         self.ne = params.ebins
         self.shape = (self.nbeam, 4, 4, self.ne)
@@ -193,6 +214,7 @@ class Context:
         if self.h:
             self.L.xf_destroy(self.h)
             self.h = None
+        _LIVE.discard(self)
 
     def __enter__(self):
         return self
@@ -201,6 +223,8 @@ class Context:
         self.close()
 
     def __del__(self):
+        if sys.is_finalizing():        # the exit handler has closed what it could
+            return
         try:
             self.close()
         except Exception:
@@ -337,13 +361,23 @@ class Ensemble:
         check(self.L.xf_ensemble_create(arr, n, ctypes.byref(self.opts), ctypes.byref(h)))
         self.h = h
         self.n = n
+        _LIVE.add(self)
 
     def close(self):
         if self.h:
             self.L.xf_ensemble_destroy(self.h)
             self.h = None
+        _LIVE.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -273,6 +273,9 @@ struct xf_ctx {
 };
 
 // ---------------------------------------------------------------- shapes
+// Scratch (register spills) a cooperative launch may use over its whole grid.
+constexpr double XF_COOP_SCRATCH_MAX = 32.0 * 1048576.0;
+
 static int max_bpt(int ne, int cpt) {
     int b = (1024 * cpt) / ne;
     return std::max(1, std::min(128, b));
@@ -358,6 +361,23 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     if (rc) return rc;
     if (grid > maxb)
         return xf_fail(XF_EINVAL, "shape needs %d co-resident workgroups, device holds %d", grid, maxb);
+    {
+        // Every workgroup of the launch must be resident at once (the
+        // exchanges wait for all of them), and the occupancy figure above
+        // counts registers and LDS only.  A kernel that spills also needs
+        // scratch memory for every wave in flight, which the runtime
+        // provisions per dispatch against a per-device limit and may meet by
+        // holding waves back: then some workgroups never start and the
+        // others' exchange waits expire (XF_EBARRIER).  Shapes whose scratch
+        // for the whole grid exceeds XF_COOP_SCRATCH_MAX are refused.
+        int vg = 0, spill = 0;
+        xf_step_func_info(Lq, E->geom, &vg, &spill);
+        const double need = (double)spill * 64.0 * (double)grid * (double)(L.block / 64);
+        if (spill > 0 && need > XF_COOP_SCRATCH_MAX)
+            return xf_fail(XF_EINVAL, "shape spills %d B per lane: %.0f MiB of scratch for %d workgroups x %d threads "
+                                      "exceeds the %.0f MiB a cooperative launch may use",
+                           spill, need / 1048576.0, grid, L.block, XF_COOP_SCRATCH_MAX / 1048576.0);
+    }
     L.wbeams = Lq.wbeams;
     if (!commit) {
         L.grid = grid;
@@ -417,12 +437,14 @@ static int choose_shape(xf_ensemble* E) {
             int vg = 0, spill = 0;
             int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
             if (kmax <= 0) continue;
-            // default shape without tuning: resident only when (almost) spill-free
-            if (spill > 64 && mode != 2) continue;
             int bpt = max_bpt(ne_max, cpt);
             while (bpt > 1 && block_for(bpt, ne_max, cpt) > kmax) --bpt;
             L.block = o.block > 0 ? std::min(o.block, kmax) : block_for(bpt, ne_max, cpt);
             if (L.block > kmax) continue;
+            // default shape without tuning: resident only when (almost)
+            // spill-free (the kernel for this block)
+            if (xf_step_func_info(L, E->geom, &vg, &spill) <= 0) continue;
+            if (spill > 64 && mode != 2) continue;
             L.bpt = 0;
             if (plan_shape(E, L, false) == XF_OK) return plan_shape(E, L, true);
         }
@@ -634,20 +656,32 @@ static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_m
                 int vg = 0, spill = 0;
                 int kmax = xf_step_func_info(L, E->geom, &vg, &spill);
                 if (kmax <= 0 || L.block > kmax) continue;
+                // a shape the planner refuses (does not fit the device, or
+                // would need too much scratch) is not a candidate; a shape
+                // that was launched and failed ends the tuning with its error
                 XfLaunch Lp = L;
                 if (plan_shape(E, Lp, true) != XF_OK) continue;
                 if ((rc = restore())) break;
                 double ms = 0;
-                if (ens_launch(E, 2, nullptr, &ms) != XF_OK) continue;
-                if (ens_launch(E, steps, nullptr, &ms) != XF_OK) continue;
-                bool good = true;
+                if ((rc = ens_launch(E, 2, nullptr, &ms)) || (rc = ens_launch(E, steps, nullptr, &ms))) {
+                    rc = xf_fail(rc, "autotune: shape mode %d, cpt %d, block %d, bpt %d failed: %s", E->L.mode,
+                                 E->L.cpt, E->L.block, E->L.bpt, xf_last_error());
+                    break;
+                }
                 long long done = 0;
                 for (auto& in : E->inst) {
                     XfRunScalars r2;
-                    if (in->scalars(&r2) != XF_OK || r2.status) { good = false; break; }
+                    if ((rc = in->scalars(&r2))) break;
+                    if (r2.status) {
+                        rc = xf_fail(r2.status, "autotune: shape mode %d, cpt %d, block %d, bpt %d ended with status %d "
+                                                "(%s)", E->L.mode, E->L.cpt, E->L.block, E->L.bpt, r2.status,
+                                     r2.status == XF_EBARRIER ? "an exchange wait expired" : "launch error");
+                        break;
+                    }
                     done = std::max<long long>(done, r2.steps_call);
                 }
-                if (!good || done <= 0) continue;
+                if (rc) break;
+                if (done <= 0) continue;
                 const double per = ms / (double)done;
                 if (per < best_t) {
                     best_t = per;

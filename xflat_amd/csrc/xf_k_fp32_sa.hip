@@ -1,14 +1,16 @@
-// Instantiations of the fused stepper: FP32 state and arithmetic, SA geometry (one
-// trajectory, src/Phy_SA.cpp; the whole state is one tile of one workgroup).
+// Instantiations of the fused stepper for workgroups of up to 512 threads:
+// fp32 state/arithmetic, SA geometry (up to 256 VGPRs per wave; mode 0
+// keeps the error remainders and two increments in registers from passes
+// A/B to C, modes 1, 2 and 4 recompute them in pass C).
 #include "xf_step_kernel.h"
 
 XfStepFn xf_pick_fp32_sa(int ns, int cpt, int mode) {
-    if (ns == 2 && cpt == 1 && mode == 1) return k_step<float, float, 2, XF_SA, 1, 1, 1024>;
-    if (ns == 2 && cpt == 1 && mode == 2) return k_step<float, float, 2, XF_SA, 1, 2, 1024>;
-    if (ns == 2 && cpt == 1 && mode == 4) return k_step<float, float, 2, XF_SA, 1, 4, 1024>;
-    if (ns == 2 && cpt == 2 && mode == 1) return k_step<float, float, 2, XF_SA, 2, 1, 1024>;
-    if (ns == 2 && cpt == 2 && mode == 2) return k_step<float, float, 2, XF_SA, 2, 2, 1024>;
-    if (ns == 2 && cpt == 2 && mode == 4) return k_step<float, float, 2, XF_SA, 2, 4, 1024>;
+    if (ns == 2 && cpt == 1 && mode == 1) return k_step<float, float, 2, XF_SA, 1, 1, 512>;
+    if (ns == 2 && cpt == 1 && mode == 2) return k_step<float, float, 2, XF_SA, 1, 2, 512>;
+    if (ns == 2 && cpt == 1 && mode == 4) return k_step<float, float, 2, XF_SA, 1, 4, 512>;
+    if (ns == 2 && cpt == 2 && mode == 1) return k_step<float, float, 2, XF_SA, 2, 1, 512>;
+    if (ns == 2 && cpt == 2 && mode == 2) return k_step<float, float, 2, XF_SA, 2, 2, 512>;
+    if (ns == 2 && cpt == 2 && mode == 4) return k_step<float, float, 2, XF_SA, 2, 4, 512>;
     if (ns == 2 && cpt == 4 && mode == 1) return k_step<float, float, 2, XF_SA, 4, 1, 512>;
     if (ns == 2 && cpt == 4 && mode == 2) return k_step<float, float, 2, XF_SA, 4, 2, 512>;
     if (ns == 2 && cpt == 4 && mode == 4) return k_step<float, float, 2, XF_SA, 4, 4, 512>;

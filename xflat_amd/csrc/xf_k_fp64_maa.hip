@@ -1,4 +1,7 @@
-// Instantiations of the fused stepper: fp64 state/arithmetic, MAA geometry.
+// Instantiations of the fused stepper for workgroups of up to 512 threads:
+// fp64 state/arithmetic, MAA geometry (up to 256 VGPRs per wave; mode 0
+// keeps the error remainders and two increments in registers from passes
+// A/B to C, modes 1, 2 and 4 recompute them in pass C).
 #include "xf_step_kernel.h"
 
 XfStepFn xf_pick_fp64_maa(int ns, int cpt, int mode) {

@@ -1,19 +1,21 @@
-// Instantiations of the fused stepper: FP64 state, FP32 increments, SA geometry (one
-// trajectory, src/Phy_SA.cpp; the whole state is one tile of one workgroup).
+// Instantiations of the fused stepper for workgroups of up to 512 threads:
+// fp64 state, fp32 increments, SA geometry (up to 256 VGPRs per wave; mode 0
+// keeps the error remainders and two increments in registers from passes
+// A/B to C, modes 1, 2 and 4 recompute them in pass C).
 #include "xf_step_kernel.h"
 
 XfStepFn xf_pick_mixed_sa(int ns, int cpt, int mode) {
-    if (ns == 2 && cpt == 1 && mode == 1) return k_step<double, float, 2, XF_SA, 1, 1, 1024>;
-    if (ns == 2 && cpt == 1 && mode == 2) return k_step<double, float, 2, XF_SA, 1, 2, 1024>;
-    if (ns == 2 && cpt == 1 && mode == 4) return k_step<double, float, 2, XF_SA, 1, 4, 1024>;
-    if (ns == 2 && cpt == 2 && mode == 1) return k_step<double, float, 2, XF_SA, 2, 1, 1024>;
-    if (ns == 2 && cpt == 2 && mode == 2) return k_step<double, float, 2, XF_SA, 2, 2, 1024>;
-    if (ns == 2 && cpt == 2 && mode == 4) return k_step<double, float, 2, XF_SA, 2, 4, 1024>;
+    if (ns == 2 && cpt == 1 && mode == 1) return k_step<double, float, 2, XF_SA, 1, 1, 512>;
+    if (ns == 2 && cpt == 1 && mode == 2) return k_step<double, float, 2, XF_SA, 1, 2, 512>;
+    if (ns == 2 && cpt == 1 && mode == 4) return k_step<double, float, 2, XF_SA, 1, 4, 512>;
+    if (ns == 2 && cpt == 2 && mode == 1) return k_step<double, float, 2, XF_SA, 2, 1, 512>;
+    if (ns == 2 && cpt == 2 && mode == 2) return k_step<double, float, 2, XF_SA, 2, 2, 512>;
+    if (ns == 2 && cpt == 2 && mode == 4) return k_step<double, float, 2, XF_SA, 2, 4, 512>;
     if (ns == 2 && cpt == 4 && mode == 1) return k_step<double, float, 2, XF_SA, 4, 1, 512>;
     if (ns == 2 && cpt == 4 && mode == 2) return k_step<double, float, 2, XF_SA, 4, 2, 512>;
     if (ns == 2 && cpt == 4 && mode == 4) return k_step<double, float, 2, XF_SA, 4, 4, 512>;
-    if (ns == 4 && cpt == 1 && mode == 1) return k_step<double, float, 4, XF_SA, 1, 1, 1024>;
-    if (ns == 4 && cpt == 1 && mode == 2) return k_step<double, float, 4, XF_SA, 1, 2, 1024>;
-    if (ns == 4 && cpt == 1 && mode == 4) return k_step<double, float, 4, XF_SA, 1, 4, 1024>;
+    if (ns == 4 && cpt == 1 && mode == 1) return k_step<double, float, 4, XF_SA, 1, 1, 512>;
+    if (ns == 4 && cpt == 1 && mode == 2) return k_step<double, float, 4, XF_SA, 1, 2, 512>;
+    if (ns == 4 && cpt == 1 && mode == 4) return k_step<double, float, 4, XF_SA, 1, 4, 512>;
     return nullptr;
 }

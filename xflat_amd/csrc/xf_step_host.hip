@@ -7,30 +7,56 @@
 constexpr size_t XF_LDS_MAX = 160 * 1024;        // LDS per workgroup on gfx950
 
 using XfStepFn = void (*)(const XfInstDev*, const int*, int);
+// instantiation tables (xf_k_*.hip): workgroups of up to 512 threads, and
+// (FP32 and mixed, 1 or 2 cells per lane) of 513..1024 threads
 XfStepFn xf_pick_fp64_ma(int, int, int);
 XfStepFn xf_pick_fp64_maa(int, int, int);
+XfStepFn xf_pick_fp64_sa(int, int, int);
 XfStepFn xf_pick_mixed_ma(int, int, int);
 XfStepFn xf_pick_mixed_maa(int, int, int);
+XfStepFn xf_pick_mixed_sa(int, int, int);
 XfStepFn xf_pick_fp32_ma(int, int, int);
 XfStepFn xf_pick_fp32_maa(int, int, int);
-XfStepFn xf_pick_fp64_sa(int, int, int);
-XfStepFn xf_pick_mixed_sa(int, int, int);
 XfStepFn xf_pick_fp32_sa(int, int, int);
+XfStepFn xf_pick_mixed_ma_wide(int, int, int);
+XfStepFn xf_pick_mixed_maa_wide(int, int, int);
+XfStepFn xf_pick_mixed_sa_wide(int, int, int);
+XfStepFn xf_pick_fp32_ma_wide(int, int, int);
+XfStepFn xf_pick_fp32_maa_wide(int, int, int);
+XfStepFn xf_pick_fp32_sa_wide(int, int, int);
 
+// The kernel for a shape: the 512-thread instantiation (up to 256 VGPRs per
+// wave) when the block fits it, else the 1024-thread one (at most 128 VGPRs).
+// A shape whose block is not chosen yet (0) gets the widest kernel, so that
+// its thread bound is the one a planner sizes blocks against.
 static XfStepFn pick(const XfLaunch& L, int geom) {
-    if (geom == XF_SA) {
-        switch (L.precision) {
-            case XF_FP64: return xf_pick_fp64_sa(L.ns, L.cpt, L.mode);
-            case XF_FP32: return xf_pick_fp32_sa(L.ns, L.cpt, L.mode);
-            default: return xf_pick_mixed_sa(L.ns, L.cpt, L.mode);
-        }
-    }
-    bool maa = geom == XF_MAA;
+    const int ns = L.ns, cpt = L.cpt, mode = L.mode;
+    XfStepFn narrow = nullptr, wide = nullptr;
     switch (L.precision) {
-        case XF_FP64: return maa ? xf_pick_fp64_maa(L.ns, L.cpt, L.mode) : xf_pick_fp64_ma(L.ns, L.cpt, L.mode);
-        case XF_FP32: return maa ? xf_pick_fp32_maa(L.ns, L.cpt, L.mode) : xf_pick_fp32_ma(L.ns, L.cpt, L.mode);
-        default: return maa ? xf_pick_mixed_maa(L.ns, L.cpt, L.mode) : xf_pick_mixed_ma(L.ns, L.cpt, L.mode);
+        case XF_FP64:
+            narrow = geom == XF_SA    ? xf_pick_fp64_sa(ns, cpt, mode)
+                     : geom == XF_MAA ? xf_pick_fp64_maa(ns, cpt, mode)
+                                      : xf_pick_fp64_ma(ns, cpt, mode);
+            break;
+        case XF_FP32:
+            narrow = geom == XF_SA    ? xf_pick_fp32_sa(ns, cpt, mode)
+                     : geom == XF_MAA ? xf_pick_fp32_maa(ns, cpt, mode)
+                                      : xf_pick_fp32_ma(ns, cpt, mode);
+            wide = geom == XF_SA    ? xf_pick_fp32_sa_wide(ns, cpt, mode)
+                   : geom == XF_MAA ? xf_pick_fp32_maa_wide(ns, cpt, mode)
+                                    : xf_pick_fp32_ma_wide(ns, cpt, mode);
+            break;
+        default:
+            narrow = geom == XF_SA    ? xf_pick_mixed_sa(ns, cpt, mode)
+                     : geom == XF_MAA ? xf_pick_mixed_maa(ns, cpt, mode)
+                                      : xf_pick_mixed_ma(ns, cpt, mode);
+            wide = geom == XF_SA    ? xf_pick_mixed_sa_wide(ns, cpt, mode)
+                   : geom == XF_MAA ? xf_pick_mixed_maa_wide(ns, cpt, mode)
+                                    : xf_pick_mixed_ma_wide(ns, cpt, mode);
+            break;
     }
+    if (L.block > 0 && L.block <= 512) return narrow;
+    return wide ? wide : narrow;
 }
 
 bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nullptr; }
