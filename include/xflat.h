@@ -195,6 +195,13 @@ int  xf_state_upload(xf_ctx* ctx, const double* psi, double r, double dr);
 int  xf_state_init_pure(xf_ctx* ctx);            /* pure flavour, r = R0, dr = dr= */
 int  xf_state_download(xf_ctx* ctx, double* psi, double* r, double* dr);
 int  xf_state_counters(const xf_ctx* ctx, int64_t* total, int64_t* accepted);
+/* Moves the end of the run, Rn= (Util::setRn, the loop condition r < Rn of
+ * Nmr.cpp:130; the step that would pass it is clamped to end on it,
+ * Nmr.cpp:249), so that a run can stop exactly on any radius and go on from
+ * there.  A run that ended on the old Rn has dr = 0; it restarts with the
+ * length of its last accepted step, the dr the reference stores with a dump
+ * and resumes from (Nmr.cpp:238, IO_f.cpp:197-250).                        */
+int  xf_set_rn(xf_ctx* ctx, double rn);
 int  xf_state_species(const xf_ctx* ctx);        /* 2 or 4 in use           */
 /* Attempt up to max_steps radial steps (fewer if r reaches Rn or a stop
  * condition fires).  log (optional) receives one record per attempt.       */
@@ -266,6 +273,8 @@ int  xf_ensemble_run_graph(xf_ensemble* ens, int launches, int max_steps, xf_run
 /* Launch-shape tuner over all instances (as xf_autotune; states restored). */
 int  xf_ensemble_autotune(xf_ensemble* ens, int steps, xf_opts* best, double* best_ms_per_step);
 int  xf_ensemble_cells(const xf_ensemble* ens, int64_t* cells);
+/* xf_set_rn for instance i of an ensemble (each instance stops on its own Rn). */
+int  xf_ensemble_set_rn(xf_ensemble* ens, int i, double rn);
 int  xf_ensemble_launch(const xf_ensemble* ens, int* mode, int* cpt, int* block, int* grid);
 
 #ifdef __cplusplus

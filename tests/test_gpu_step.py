@@ -54,23 +54,51 @@ def _run_all(ctx, n, chunk=None):
     return np.concatenate(logs)
 
 
+def _run_to_records(ctx, g):
+    """Runs the golden case's attempts in chunks that end at each kept record
+    (the state after accepted step psi_idx[j], i.e. after attempt
+    it[psi_idx[j]]), checking P_ee and Pbar_ee against the record there:
+    the reference's output radii (its dump hook, Nmr.cpp:238).  Returns the
+    concatenated step log and the worst |dP| over the records."""
+    logs, done, worst = [], 0, 0.0
+    ends = [int(g["it"][k]) + 1 for k in g["psi_idx"]]
+    for j, end in enumerate(ends):
+        while done < end:
+            info, rec = ctx.run(end - done, log=True)
+            logs.append(rec)
+            done += info.steps
+            assert info.steps > 0
+        psi, r, _ = ctx.download()
+        k = int(g["psi_idx"][j])
+        assert abs(r - g["r"][k]) <= R_RTOL * g["r"][k], (j, r, g["r"][k])
+        pe, pb = _pee(psi)
+        ge, gb = _pee(g["psi"][j])
+        d = max(np.abs(pe - ge).max(), np.abs(pb - gb).max())
+        assert d <= PEE_TOL, (j, d)
+        worst = max(worst, d)
+    total = int(g["total"])
+    if done < total:
+        logs.append(_run_all(ctx, total - done))
+    return np.concatenate(logs), worst
+
+
 @pytest.mark.parametrize("name", golden_files())
 def test_fused_fp64_matches_reference(name):
+    """The FP64 fused stepper against a reference run: the same accepted-step
+    sequence, r and dr at every accepted step, and P_ee / Pbar_ee within
+    1e-9 at every kept record (the output radii)."""
     capi = _capi()
     g, p = _golden(name)
     print(f"[fused fp64] {name}", flush=True)
     with capi.Context(p, precision=capi.XF_FP64) as ctx:
         ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
-        rec = _run_all(ctx, int(g["total"]))
+        rec, worst = _run_to_records(ctx, g)
         acc = np.nonzero(rec["accepted"])[0]
         assert np.array_equal(acc, g["it"]), "accepted-step sequence differs"
         np.testing.assert_allclose(rec["dr"][acc], g["dr"], rtol=DR_RTOL, atol=0)
         np.testing.assert_allclose((rec["r"] + rec["dr"])[acc], g["r"], rtol=R_RTOL, atol=0)
         psi, r, dr = ctx.download()
-        pe, pb = _pee(psi)
-        ge, gb = _pee(g["psi"][-1])
-        assert np.abs(pe - ge).max() <= PEE_TOL
-        assert np.abs(pb - gb).max() <= PEE_TOL
+        print(f"[fused fp64] {name}: {len(g['psi_idx'])} records, max |dP| {worst:.2e}", flush=True)
         # the complex phases see the ~1e-12 km radius differences of the step
         # sequence; P_ee above is the contract
         assert np.abs(psi - g["psi"][-1]).max() <= 1e-7
@@ -83,13 +111,10 @@ def test_fused_fp64_four_species(name):
     with capi.Context(p, precision=capi.XF_FP64, species=4) as ctx:
         assert ctx.species() == 4
         ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
-        rec = _run_all(ctx, int(g["total"]))
+        rec, worst = _run_to_records(ctx, g)
         acc = np.nonzero(rec["accepted"])[0]
         assert np.array_equal(acc, g["it"])
         psi, *_ = ctx.download()
-        pe, pb = _pee(psi)
-        ge, gb = _pee(g["psi"][-1])
-        assert np.abs(pe - ge).max() <= PEE_TOL and np.abs(pb - gb).max() <= PEE_TOL
         assert np.abs(psi - g["psi"][-1]).max() <= 1e-7
 
 

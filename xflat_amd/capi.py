@@ -95,6 +95,7 @@ SIGNATURES = {
     "xf_state_download": (_I, [_P, _DP, _DP, _DP]),
     "xf_state_counters": (_I, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "xf_state_species": (_I, [_P]),
+    "xf_set_rn": (_I, [_P, _D]),
     "xf_run_steps": (_I, [_P, _I, ctypes.POINTER(XfStop), _P, ctypes.POINTER(XfRunInfo)]),
     "xf_run_steps_noalloc": (_I, [_P, _I, ctypes.POINTER(XfStop), _P, ctypes.POINTER(XfRunInfo)]),
     "xf_run_graph": (_I, [_P, _I, _I, ctypes.POINTER(XfRunInfo)]),
@@ -113,6 +114,7 @@ SIGNATURES = {
     "xf_ensemble_run_graph": (_I, [_P, _I, _I, ctypes.POINTER(XfRunInfo)]),
     "xf_ensemble_autotune": (_I, [_P, _I, _OP, _DP]),
     "xf_ensemble_cells": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "xf_ensemble_set_rn": (_I, [_P, _I, _D]),
     "xf_ensemble_launch": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     # include/xflat_io.h (host-side NetCDF output and resume)
     "xf_snapshot_create": (_I, [_PP, ctypes.c_char_p, ctypes.POINTER(_P)]),
@@ -252,6 +254,23 @@ class Context:
 
     def species(self) -> int:
         return self.L.xf_state_species(self.h)
+
+    def set_rn(self, rn: float):
+        """Moves the end radius Rn (runs stop exactly on it)."""
+        check(self.L.xf_set_rn(self.h, rn))
+
+    def run_to(self, rn: float, max_attempts: int = 1 << 20, chunk: int = 4000) -> int:
+        """Runs until r == rn exactly (Rn moved to rn); returns the attempts."""
+        self.set_rn(rn)
+        done = 0
+        while done < max_attempts:
+            info, _ = self.run(min(chunk, max_attempts - done))
+            done += info.steps
+            if info.stop_reason == 1:
+                return done
+            if info.steps == 0:
+                break
+        raise RuntimeError(f"r = {rn} not reached in {done} attempts")
 
     def run(self, max_steps: int, log: bool = False, stop: XfStop | None = None):
         rec = np.zeros(max(max_steps, 1), dtype=STEP_DTYPE) if log else None
@@ -410,6 +429,25 @@ class Ensemble:
         infos = (XfRunInfo * self.n)()
         check(self.L.xf_ensemble_run_graph(self.h, launches, max_steps, infos))
         return list(infos)
+
+    def set_rn(self, i: int, rn: float):
+        check(self.L.xf_ensemble_set_rn(self.h, i, rn))
+
+    def run_to(self, radii, max_attempts: int = 1 << 20, chunk: int = 2000):
+        """Every instance i runs until r == radii[i] exactly; returns the
+        attempts of each."""
+        for i, rn in enumerate(radii):
+            self.set_rn(i, rn)
+        done = [0] * self.n
+        left = max_attempts
+        while left > 0:
+            infos = self.run(min(chunk, left))
+            left -= max(x.steps for x in infos)
+            for i, x in enumerate(infos):
+                done[i] += x.steps
+            if all(x.stop_reason == 1 for x in infos):
+                return done
+        raise RuntimeError("not every instance reached its radius")
 
     def cells(self) -> int:
         c = ctypes.c_int64()

@@ -273,7 +273,8 @@ struct xf_ctx {
 };
 
 // ---------------------------------------------------------------- shapes
-// Scratch (register spills) a cooperative launch may use over its whole grid.
+// Scratch (register spills) a stepper launch, whose workgroups must all be
+// resident at once, may use over its whole grid.
 constexpr double XF_COOP_SCRATCH_MAX = 32.0 * 1048576.0;
 
 static int max_bpt(int ne, int cpt) {
@@ -375,7 +376,7 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
         const double need = (double)spill * 64.0 * (double)grid * (double)(L.block / 64);
         if (spill > 0 && need > XF_COOP_SCRATCH_MAX)
             return xf_fail(XF_EINVAL, "shape spills %d B per lane: %.0f MiB of scratch for %d workgroups x %d threads "
-                                      "exceeds the %.0f MiB a cooperative launch may use",
+                                      "exceeds the %.0f MiB a co-resident launch may use",
                            spill, need / 1048576.0, grid, L.block, XF_COOP_SCRATCH_MAX / 1048576.0);
     }
     L.wbeams = Lq.wbeams;
@@ -840,6 +841,32 @@ int xf_state_counters(const xf_ctx* ctx, int64_t* total, int64_t* accepted) {
 
 int xf_state_species(const xf_ctx* ctx) { return ctx ? ctx->ens->inst[0]->ns : 0; }
 
+// Rn of one instance; a run that ended on the old Rn (dr = 0) restarts with
+// the length of its last accepted step.  The descriptor copy on the device is
+// refreshed: xf_run_steps copies it before every launch, an ensemble here.
+static int inst_set_rn(xf_ensemble* E, size_t i, double rn) {
+    if (!(rn > 0.) || !std::isfinite(rn)) return xf_fail(XF_EINVAL, "Rn = %g", rn);
+    XfInst* in = E->inst[i].get();
+    XfRunScalars rs;
+    int rc = in->scalars(&rs);
+    if (rc) return rc;
+    if (rs.dr == 0. && rn > rs.r && rs.last_dr > 0.) {
+        rs.dr = std::min(rs.last_dr, rn - rs.r);
+        HIPCHK(hipMemcpy(in->d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
+    }
+    in->p.Rn = rn;
+    in->dev.Rn = rn;
+    if (E->shaped && E->d_insts) HIPCHK(hipMemcpy(E->d_insts + i, &in->dev, sizeof(XfInstDev), hipMemcpyHostToDevice));
+    return XF_OK;
+}
+
+int xf_set_rn(xf_ctx* ctx, double rn) {
+    if (!ctx) return xf_fail(XF_EINVAL, "xf_set_rn: null ctx");
+    int rc = inst_set_rn(ctx->ens, 0, rn);
+    if (rc == XF_OK) ctx->p.Rn = rn;
+    return rc;
+}
+
 int xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop, xf_step_rec* log, xf_run_info* info) {
     if (!ctx || max_steps < 0) return xf_fail(XF_EINVAL, "xf_run_steps: bad arguments");
     xf_ensemble* E = ctx->ens;
@@ -1090,6 +1117,10 @@ int xf_ensemble_cells(const xf_ensemble* ens, int64_t* cells) {
     for (auto& in : ens->inst) c += in->ncells();
     *cells = c;
     return XF_OK;
+}
+int xf_ensemble_set_rn(xf_ensemble* ens, int i, double rn) {
+    if (!ens || i < 0 || i >= (int)ens->inst.size()) return xf_fail(XF_EINVAL, "xf_ensemble_set_rn: bad instance");
+    return inst_set_rn(ens, (size_t)i, rn);
 }
 int xf_ensemble_launch(const xf_ensemble* ens, int* mode, int* cpt, int* block, int* grid) {
     if (!ens || !ens->shaped) return xf_fail(XF_EINVAL, "no launch shape yet");

@@ -1,5 +1,5 @@
 // xf_step_host.hip -- launch helpers of the fused stepper: kernel choice,
-// LDS size, co-residency, cooperative launch.
+// LDS size, co-residency, launch.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include "xf_internal.h"
@@ -129,10 +129,19 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
         hipError_t e = hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
     }
+    // An ordinary launch of a grid that the planner sized to be co-resident
+    // (xf_step_max_blocks_geom).  The kernel needs every workgroup resident
+    // at once but no device-wide synchronisation resource, so it does not
+    // take the runtime's cooperative-launch path, whose device resource one
+    // process at a time can hold: a second process (a host program started by
+    // a process that has run steppers) would be refused.  If the grid is not
+    // resident after all (another kernel holds CUs), the exchanges' bounded
+    // waits end the launch with XF_EBARRIER.
     void* args[] = {(void*)&d_insts, (void*)&d_blk_inst, (void*)&max_steps};
-    hipError_t e = hipLaunchCooperativeKernel((const void*)f, dim3(L.grid), dim3(L.block), args, smem, (hipStream_t)stream);
+    (void)hipGetLastError();
+    hipError_t e = hipLaunchKernel((const void*)f, dim3(L.grid), dim3(L.block), args, smem, (hipStream_t)stream);
     if (e != hipSuccess)
-        return xf_fail(XF_EHIP, "cooperative launch (grid %d x %d, %zu B LDS): %s", L.grid, L.block, smem, hipGetErrorString(e));
+        return xf_fail(XF_EHIP, "stepper launch (grid %d x %d, %zu B LDS): %s", L.grid, L.block, smem, hipGetErrorString(e));
     return XF_OK;
 }
 
@@ -159,8 +168,8 @@ int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream) {
 
 // The radial loop as one graph: the stepper, then launches - 1 pairs (rows
 // cleared with the status kept, stepper), each node after the previous one.
-// The stepper nodes carry the cooperative launch attribute, as
-// hipLaunchCooperativeKernel does for a direct launch.  No memset node: the
+// The stepper nodes are ordinary kernel nodes, as the direct launch is (the
+// planner sized the grid to be co-resident).  No memset node: the
 // rows are cleared by a kernel.
 int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int ninst,
                         int max_steps, int launches, hipGraphExec_t* exec) {
@@ -192,8 +201,6 @@ int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
     pp.blockDim = dim3(256);
     pp.sharedMemBytes = 0;
     pp.kernelParams = prep_args;
-    hipLaunchAttributeValue coop{};
-    coop.cooperative = 1;
     hipGraphNode_t prev = nullptr;
     for (int k = 0; k < launches && e == hipSuccess; ++k) {
         hipGraphNode_t node = nullptr;
@@ -203,8 +210,6 @@ int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
             prev = node;
         }
         e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &sp);
-        if (e != hipSuccess) break;
-        e = hipGraphKernelNodeSetAttribute(node, hipKernelNodeAttributeCooperative, &coop);
         prev = node;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);

@@ -2,7 +2,7 @@
 // xf_step_kernel.h -- the fused multi-angle evolution step for CDNA4 (gfx950).
 // Included by the xf_k_*.hip translation units, each instantiating a subset.
 //
-// One cooperative launch attempts up to max_steps radial steps of
+// One persistent launch (every workgroup resident) attempts up to max_steps radial steps of
 // nmr::evolutionLoop (src/Nmr.cpp:125-252) for one or many independent
 // instances.  Each instance owns a contiguous range of workgroups; a
 // workgroup owns "tiles" of whole beams (theta[, phi]) with all their energy

@@ -65,10 +65,13 @@ def pure_psi(nbeams: int, ne: int) -> np.ndarray:
     return psi
 
 
-def azimuthal_psi(nth: int, nph: int, ne: int, seed: int, amp: float = 1e-7, mmax: int = 4) -> np.ndarray:
-    """MAA initial state with an azimuthal-symmetry-breaking perturbation:
-    eps(phi) = amp * sum_{m=1..mmax} exp(i (m phi + phi_m)), phi_m = 2 pi u_m
-    (u_m the m-th output of the splitmix64 stream `seed`, separately for the
+def azimuthal_psi(nth: int, nph: int, ne: int, seed: int, amp: float = 1e-7, mmax: int = 4,
+                  form: str = "cos") -> np.ndarray:
+    """MAA initial state with an azimuthal-symmetry-breaking perturbation of
+    the multi-azimuthal workload: eps(phi) = amp * sum_{m=1..mmax}
+    cos(m phi + phi_m), real-valued (form "cos"), or
+    amp * sum_m exp(i (m phi + phi_m)) (form "exp"); phi_m = 2 pi u_m (u_m
+    the m-th output of the splitmix64 stream `seed`, separately for the
     neutrinos and the antineutrinos), phi = (p + 1/2) 2 pi / Pbins the
     reference's azimuth grid (Phy_MAA.cpp:177-185); the same for every polar
     bin and energy.  Partner columns as in seeded_psi."""
@@ -76,7 +79,12 @@ def azimuthal_psi(nth: int, nph: int, ne: int, seed: int, amp: float = 1e-7, mma
     psi = np.zeros((nth * nph, 4, 4, ne))
     for sp in (0, 1):
         ph = 2.0 * np.pi * uniform01(seed, np.arange(mmax) + sp * mmax)
-        eps = amp * sum(np.exp(1j * (m * phi + ph[m - 1])) for m in range(1, mmax + 1))
+        if form == "cos":
+            eps = amp * sum(np.cos(m * phi + ph[m - 1]) for m in range(1, mmax + 1)) + 0j
+        elif form == "exp":
+            eps = amp * sum(np.exp(1j * (m * phi + ph[m - 1])) for m in range(1, mmax + 1))
+        else:
+            raise ValueError(form)
         er = np.repeat(np.tile(eps.real, nth)[:, None], ne, axis=1)
         ei = np.repeat(np.tile(eps.imag, nth)[:, None], ne, axis=1)
         a = np.sqrt(1.0 - (er * er + ei * ei))
