@@ -1,0 +1,115 @@
+"""The boundary proved from the reference's side: the reference's own sources,
+built by oracle/Makefile with one translation unit exchanged for a binding
+over the C-ABI (xflat_amd/bind/) and linked with libxflat.so, reproduce the
+reference's own runs (the golden fixtures: every accepted step's attempt
+index, r and dr, and psi at the kept records).
+
+- xflat_ref_<module>_gpu: xflat_amd/bind/Nmr_gpu.cpp in place of src/Nmr.cpp
+  (nmr::evolutionLoop on the GPU, xf_run_steps), the reference's Phy module,
+  NBGroup, Parser and its IO interface (Fio.h, here the raw recorder) as they
+  are.
+- xflat_ref_<module>_gpuphy: xflat_amd/bind/Phy_gpu.cpp in place of
+  src/Phy_<module>.cpp (every Phy.h call on the GPU) under the reference's
+  unchanged src/Nmr.cpp.
+
+FP64 bar: the same accepted-step sequence, dr to 2e-6 relative at every
+accepted step and the distance advanced (r - R0) to the same, max |dP_ee| and
+|dPbar_ee| <= 1e-9 at every kept record.  (err = sqrt(max sum_e fv
+|psi4 - psi1|^2) cancels O(1) values that agree to ~1e-9, so an ulp of the
+device's sin/cos against glibc's in the 1:1 mirror kernels moves dr by up to
+~1e-6 relative; the fused stepper's propagator series stay at ~1e-11.)
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+PEE_TOL = 1e-9
+DR_RTOL = 2e-6
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+CASES = [("ma_r10_ih.npz", "ma"), ("ma_r50_ih.npz", "ma"), ("sa_r10_ih.npz", "sa"), ("maa_r10.npz", "maa"),
+         ("maa_pert.npz", "maa_pert")]
+
+
+def _config(g, every_step_dump):
+    txt = str(g["config"])
+    if every_step_dump:
+        # snapshot stream on, due after every accepted step: the GPU loop then
+        # hands the IO module every accepted step, as Nmr.cpp:238 does
+        for key, val in (("dumpMode=", "1"), ("itr_step1=", "1"), ("r_step1=", "1e9"), ("t_step1=", "100000000")):
+            txt = re.sub(rf"^{re.escape(key)}.*$", f"{key} {val}", txt, flags=re.M)
+    return txt
+
+
+def _read_records(path, shape):
+    raw = np.fromfile(path, dtype=np.uint8)
+    per = int(np.prod(shape[1:]))
+    pos = 32
+    its, rs, drs, psis = [], [], [], []
+    while pos < raw.size:
+        it = raw[pos:pos + 8].view(np.int64)[0]
+        r, dr = raw[pos + 8:pos + 24].view(np.float64)
+        has = raw[pos + 24:pos + 32].view(np.int64)[0]
+        pos += 32
+        its.append(int(it)); rs.append(r); drs.append(dr)
+        if has:
+            psis.append(raw[pos:pos + 8 * per * shape[0]].view(np.float64).reshape(shape).copy())
+            pos += 8 * per * shape[0]
+        else:
+            psis.append(None)
+    return np.array(its), np.array(rs), np.array(drs), psis
+
+
+def _pee(psi):
+    return psi[:, 0, 0] ** 2 + psi[:, 0, 1] ** 2, psi[:, 1, 0] ** 2 + psi[:, 1, 1] ** 2
+
+
+@pytest.mark.parametrize("binding", ["gpu", "gpuphy"])
+@pytest.mark.parametrize("name,module", CASES)
+def test_reference_built_with_binding_reproduces_golden(name, module, binding, tmp_path):
+    g = np.load(os.path.join(GOLDEN, name))
+    exe = os.path.join(REF, f"xflat_ref_{module}_{binding}")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (oracle/Makefile, target bind)")
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(_config(g, binding == "gpu"))
+    out = tmp_path / "rec.bin"
+    env = dict(os.environ, XFLAT_REF_OUT=str(out), XFLAT_REF_PSI_EVERY="1", OMP_NUM_THREADS="1")
+    env.pop("XFLAT_REF_INIT", None)
+    args = [exe, str(cfg)]
+    nbeam, ne = g["psi0"].shape[0], g["psi0"].shape[-1]
+    if module in ("ma", "sa"):
+        init = tmp_path / "init.bin"
+        with open(init, "wb") as f:
+            np.array([nbeam * 4, 4, ne], dtype=np.int64).tofile(f)
+            np.array([float(g["r0"]), float(g["dr0"])]).tofile(f)
+            g["psi0"].astype(np.float64).tofile(f)
+        env["XFLAT_REF_INIT"] = str(init)
+        args.append(str(init))
+    print(f"[refbind] {os.path.basename(exe)} on {name}", flush=True)
+    r = subprocess.run(args, env=env, capture_output=True, text=True, cwd=str(tmp_path), timeout=300)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    m = re.search(r"Total iterations:\s*(\d+)\s+Correct iterations:\s*(\d+)", r.stdout)
+    assert m, r.stdout[-2000:]
+    assert (int(m.group(1)), int(m.group(2))) == (int(g["total"]), int(g["accepted"]))
+    its, rs, drs, psis = _read_records(str(out), (nbeam, 4, 4, ne))
+    assert np.array_equal(its, g["it"]), "accepted-step sequence differs"
+    print(f"[refbind] max relative error of r {np.abs(rs / g['r'] - 1).max():.2e}, of dr "
+          f"{np.abs(drs / g['dr'] - 1).max():.2e}", flush=True)
+    np.testing.assert_allclose(rs - float(g["r0"]), g["r"] - float(g["r0"]), rtol=DR_RTOL, atol=0)
+    np.testing.assert_allclose(drs, g["dr"], rtol=DR_RTOL, atol=0)
+    worst = 0.0
+    for j, k in enumerate(g["psi_idx"]):
+        (pe, pb), (ge, gb) = _pee(psis[int(k)]), _pee(g["psi"][j])
+        d = max(np.abs(pe - ge).max(), np.abs(pb - gb).max())
+        assert d <= PEE_TOL, (j, d)
+        worst = max(worst, d)
+    print(f"[refbind] {name} {binding}: {len(its)} accepted steps as the reference's, "
+          f"{len(g['psi_idx'])} records, max |dP| {worst:.2e}", flush=True)
