@@ -301,6 +301,8 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     int grid = 0;
     std::vector<int> nblk(E->inst.size()), bpts(E->inst.size()), tiles(E->inst.size());
     int max_bpt_used = 1;
+    L.ne = 1;                            // the per-energy LDS table holds the largest instance's bins
+    for (auto& in : E->inst) L.ne = std::max(L.ne, in->t.ne);
     for (size_t i = 0; i < E->inst.size(); ++i) {
         XfInst* in = E->inst[i].get();
         int ne = in->t.ne;

@@ -118,6 +118,7 @@ struct XfLaunch {
     int bpt;           // beams per tile
     int grid;          // workgroups (all instances)
     int wbeams;        // mode 4: largest beam count of one workgroup's tiles (records in LDS)
+    int ne;            // largest energy-bin count of the launch's instances (per-energy LDS table)
 };
 
 // Launch helpers of the fused stepper (xf_step_host.hip).

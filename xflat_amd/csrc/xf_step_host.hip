@@ -67,13 +67,18 @@ static int record_doubles(int geom) {
     return 3 + 3 * nm + 9 + 12 + 4;
 }
 
+// The kernel's dynamic LDS, in doubles, in its order: beam records, the
+// pass-C error buffer (which doubles as the publish staging of the lanes'
+// values), the per-beam error sums, the per-energy table of (6 + species) | 1
+// slots per bin, and for mode 0 with FP32 increments the kept remainder
+// coefficient sets (a float4 per cell slot and species, 16-byte aligned).
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom) {
-    const int bsize = record_doubles(geom);
-    // the pass-C error buffer doubles as the publish staging of the lanes'
-    // values (in chunks of values)
-    size_t errb = (size_t)4 * L.block * L.cpt;
+    const size_t bsize = record_doubles(geom);
+    const size_t errb = (size_t)4 * L.block * L.cpt;
     const size_t recs = L.mode == 4 ? (size_t)std::max(L.wbeams, L.bpt) : (size_t)L.bpt;   // beam records
-    return sizeof(double) * (recs * bsize + errb + 4 * (size_t)L.bpt);
+    size_t n = recs * bsize + errb + 4 * (size_t)L.bpt + (size_t)std::max(L.ne, 1) * ((6 + L.ns) | 1);
+    if (L.mode == 0 && L.precision != XF_FP64) n = ((n + 1) & ~(size_t)1) + (size_t)2 * L.cpt * L.ns * L.block;
+    return sizeof(double) * n;
 }
 
 // Mode 4: the most beam records that fit the workgroup's LDS beside the rest

@@ -438,11 +438,21 @@ __device__ __forceinline__ void st_sp(ST* __restrict__ base, long long ncells, i
     p[3 * ncells] = (ST)v.bi;
 }
 
+// Per-energy table in LDS, one record of tab_slots(NS) 8-byte slots per
+// energy bin: H_vac (h_r0, h_r1), the error weights fv of the 4 particles,
+// and the NS source weights (FP32 ones in the low half of their slot).  A
+// cell reads every field at an immediate offset from one 32-bit address.
+// The record length is odd in 8-byte units, so the lanes of a wave, which
+// hold consecutive energy bins, read 32 distinct 8-byte banks in each half
+// wave (an even length would fold them onto a few banks).
+constexpr int tab_slots(int ns) { return (6 + ns) | 1; }
+
 // Hamiltonian of species s at energy e for beam H record Hb (NBeam.cpp:602-625):
 // nu: (hv0 + Hb0, hv1 + Hb1, Hb2); anti-nu: (hv0 - Hb0, hv1 - Hb1, Hb2).
-__device__ __forceinline__ void cell_h(const XfInstDev& I, int e, int s, const double* Hb, double& h0, double& h1,
+// te: the energy's table record.
+__device__ __forceinline__ void cell_h(const double* te, int s, const double* Hb, double& h0, double& h1,
                                        double& h2) {
-    double hv0 = I.hv[e], hv1 = I.hv[I.ne + e];
+    double hv0 = te[0], hv1 = te[1];
     if (s & 1) {
         h0 = hv0 + (-Hb[0]);
         h1 = hv1 + (-Hb[1]);
@@ -457,12 +467,12 @@ __device__ __forceinline__ void cell_h(const XfInstDev& I, int e, int s, const d
 // components 0, 2, 3 (upd_nu_coef, NBeam.h:47-53, with L/<E> folded into the
 // per-energy source weights).
 template <class CT>
-__device__ __forceinline__ void src_add(const XfInstDev& I, int e, int s, const Sp<CT>& v, CT& X0, CT& X2, CT& X3) {
+__device__ __forceinline__ void src_add(const double* te, int s, const Sp<CT>& v, CT& X0, CT& X2, CT& X3) {
     CT d0, d2, d3;
     dens<CT>(v, d0, d2, d3);
     CT w;
-    if constexpr (std::is_same<CT, double>::value) w = I.srcw[s * I.ne + e];
-    else w = I.srcwf[s * I.ne + e];
+    if constexpr (std::is_same<CT, double>::value) w = te[6 + s];
+    else w = *reinterpret_cast<const float*>(te + 6 + s);
     if (s & 1) { X0 -= w * d0; X2 -= w * d2; }
     else       { X0 += w * d0; X2 += w * d2; }
     X3 += w * d3;
@@ -549,6 +559,21 @@ __device__ __forceinline__ double lane_bcast(double v) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 __device__ __forceinline__ double lane63(double v) { return lane_bcast<63>(v); }
+// A wave-uniform value held in scalar registers: the step state (r, dr,
+// counters) is read from memory or LDS, which the compiler cannot prove
+// uniform; in vector registers it would take ~20 VGPRs for the whole launch.
+__device__ __forceinline__ double uni_d(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ long long uni_ll(long long v) {
+    const unsigned long long u = (unsigned long long)v;
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
 
 // Cross-workgroup reduction of NV sums and NMAX maxima (maxima of values
 // >= 0) by an all-gather of tagged rows:
@@ -793,6 +818,28 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                                              // publish staging of the lanes' values
     const int xcap = 4 * B * CPT;                           // doubles of s_errb
     double* s_errs = s_errb + xcap;                          // [bpt][4]
+    constexpr int TS = tab_slots(NS);
+    double* s_tab = s_errs + 4 * bpt;                        // [ne][TS] per-energy table
+    // mode 0 with FP32 increments: the remainder coefficient set Q of every
+    // cell and species, kept in LDS from pass A to pass C (16-byte aligned)
+    float4* s_keep = reinterpret_cast<float4*>(smem + (((int)(s_tab - smem) + ne * TS + 1) & ~1));   // [CPT][NS][B]
+    for (int e = tid; e < ne; e += B) {
+        double* t = s_tab + e * TS;
+        t[0] = I.hv[e];
+        t[1] = I.hv[ne + e];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[2 + q] = I.errw[q * ne + e];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            if constexpr (F64) {
+                t[6 + q] = I.srcw[q * ne + e];
+            } else {
+                t[6 + q] = 0.;
+                *reinterpret_cast<float*>(t + 6 + q) = I.srcwf[q * ne + e];
+            }
+        }
+    }
+    __syncthreads();
     __shared__ int s_flag[2];
     // angular moments, uniform over the workgroup, gathered in place by the
     // exchanges: psi1 and psi3 (pass A), psi4 (pass B), and two buffers that
@@ -801,11 +848,11 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     __shared__ double s_MA[2 * NMV], s_MB[NMV], s_MC[2][NMV + 1];
 
     // uniform step state
-    double r = rs->r, dr = rs->dr;
-    long long total = rs->total, accepted = rs->accepted;
-    int cur = rs->cur;
-    double last_err = rs->last_err;
-    double last_dr = rs->last_dr;
+    double r = uni_d(rs->r), dr = uni_d(rs->dr);
+    long long total = uni_ll(rs->total), accepted = uni_ll(rs->accepted);
+    int cur = __builtin_amdgcn_readfirstlane(rs->cur);
+    double last_err = uni_d(rs->last_err);
+    double last_dr = uni_d(rs->last_dr);
     unsigned gen = 0u;                    // generation of the instance barrier
     int* status = &rs->status;
 
@@ -940,8 +987,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    if constexpr (F64) src_add<CT>(I, C.e[k], s, sp_d(C.p0[k][s]), X0, X2, X3);
-                    else src_add<CT>(I, C.e[k], s, sp_f(C.p0[k][s]), X0, X2, X3);
+                    if constexpr (F64) src_add<CT>(s_tab + C.e[k] * TS, s, sp_d(C.p0[k][s]), X0, X2, X3);
+                    else src_add<CT>(s_tab + C.e[k] * TS, s, sp_f(C.p0[k][s]), X0, X2, X3);
                 }
                 acc_moments<NM>(acc, rec + R::W + 0 * NM, (double)X0, (double)X2, (double)X3);
             }
@@ -951,7 +998,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
 
     long long steps_call = 0, acc_call = 0;
     int stop_reason = 0;
-    double hm_cur = hm_at(I, r);
+    double hm_cur = uni_d(hm_at(I, r));
     // Every exchange ends with a workgroup barrier after its last LDS access,
     // and every reader of the beam records, the staging and the moments comes
     // before one: a step starts without a barrier of its own.
@@ -1005,11 +1052,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     if (C.bl[k] < 0) continue;
                     const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
+                    const double* te = s_tab + e * TS;
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0), Y0 = CT(0), Y2 = CT(0), Y3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         double h0, h1, h2;
-                        cell_h(I, e, s, rec + R::H + 0, h0, h1, h2);
+                        cell_h(te, s, rec + R::H + 0, h0, h1, h2);
                         if constexpr (F64) {
                             double lam = lam_d(h0, h1, h2);
                             PropD P1, P0;
@@ -1022,8 +1070,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             Sp<double> a = sp_d(C.p0[k][s]);
                             Sp<double> v1 = apply_d(P1, a), v3 = apply_d(P0, a);
                             if (MODE == 0) { C.k1[k][s] = v1; C.k3[k][s] = v3; }
-                            src_add<CT>(I, e, s, v1, X0, X2, X3);
-                            src_add<CT>(I, e, s, v3, Y0, Y2, Y3);
+                            src_add<CT>(te, s, v1, X0, X2, X3);
+                            src_add<CT>(te, s, v3, Y0, Y2, Y3);
                         } else if constexpr (DBL) {
                             // dl1 = 2 dl0: U(dl1) - I = 2 D0 + D0^2
                             const float f0 = (float)h0, f1 = (float)h1, f2 = (float)h2;
@@ -1036,11 +1084,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             if (MODE == 0) {
                                 // kept for pass C: the remainder coefficients Q = R0 - R1
                                 // so far, and delta3
-                                C.k1[k][s] = pf_pack(pf_rdiff2(f0, f1, f2, cm1, sc, scr));
+                                const PropF q = pf_rdiff2(f0, f1, f2, cm1, sc, scr);
+                                s_keep[(k * NS + s) * B + tid] = make_float4(q.cm1, q.m1, q.m2, q.m3);
                                 C.k3[k][s] = d3;
                             }
-                            src_add<CT>(I, e, s, sp_add(a, d1), X0, X2, X3);
-                            src_add<CT>(I, e, s, sp_add(a, d3), Y0, Y2, Y3);
+                            src_add<CT>(te, s, sp_add(a, d1), X0, X2, X3);
+                            src_add<CT>(te, s, sp_add(a, d3), Y0, Y2, Y3);
                         } else {
                             PropF P1, P0, PR1, PR0;
                             prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], P1, PR1);
@@ -1051,11 +1100,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 // kept for pass C: the remainder coefficients Q = R0 - R1
                                 // so far, and delta3
                                 const PropF Q{0.f - PR1.cm1, 0.f - PR1.m1, 0.f - PR1.m2, 0.f - PR1.m3};
-                                C.k1[k][s] = pf_pack(pf_add(Q, PR0));
+                                const PropF q = pf_add(Q, PR0);
+                                s_keep[(k * NS + s) * B + tid] = make_float4(q.cm1, q.m1, q.m2, q.m3);
                                 C.k3[k][s] = d3;
                             }
-                            src_add<CT>(I, e, s, sp_add(a, d1), X0, X2, X3);
-                            src_add<CT>(I, e, s, sp_add(a, d3), Y0, Y2, Y3);
+                            src_add<CT>(te, s, sp_add(a, d1), X0, X2, X3);
+                            src_add<CT>(te, s, sp_add(a, d3), Y0, Y2, Y3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);        // h1 at r+dr
@@ -1108,12 +1158,13 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     if (C.bl[k] < 0) continue;
                     const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
+                    const double* te = s_tab + e * TS;
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         double h0, h1, h2, g0, g1, g2;
-                        cell_h(I, e, s, rec + R::H + 3, h0, h1, h2);      // h1 (+ Hm(r+dr))
-                        cell_h(I, e, s, rec + R::H + 6, g0, g1, g2);      // h3 (+ Hm(r+dr/2))
+                        cell_h(te, s, rec + R::H + 3, h0, h1, h2);      // h1 (+ Hm(r+dr))
+                        cell_h(te, s, rec + R::H + 6, g0, g1, g2);      // h3 (+ Hm(r+dr/2))
                         if constexpr (F64) {
                             Sp<double> a = sp_d(C.p0[k][s]);
                             PropD Pb, P4;
@@ -1123,7 +1174,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             if (MODE == 0) C.k1[k][s] = sp_avg(b, C.k1[k][s]);
                             Sp<double> v4 = apply_d(P4, a);
                             if (MODE == 0) C.k4[k][s] = v4;
-                            src_add<CT>(I, e, s, v4, X0, X2, X3);
+                            src_add<CT>(te, s, v4, X0, X2, X3);
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
                             PropF Pb, P4, PRb, PR4;
@@ -1132,10 +1183,12 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             if (MODE == 0) {
                                 // Q -= Rb, Q += R3 (the order of the recompute path)
                                 prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], Pb, PRb);
-                                C.k1[k][s] = pf_pack(pf_add(pf_sub(pf_unpack(C.k1[k][s]), PRb), PR4));
+                                const float4 o = s_keep[(k * NS + s) * B + tid];
+                                const PropF q = pf_add(pf_sub(PropF{o.x, o.y, o.z, o.w}, PRb), PR4);
+                                s_keep[(k * NS + s) * B + tid] = make_float4(q.cm1, q.m1, q.m2, q.m3);
                                 C.k4[k][s] = d4;
                             }
-                            src_add<CT>(I, e, s, sp_add(a, d4), X0, X2, X3);
+                            src_add<CT>(te, s, sp_add(a, d4), X0, X2, X3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
@@ -1202,12 +1255,13 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     }
                     const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
+                    const double* te = s_tab + e * TS;
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
                         double dsq;
                         double h0, h1, h2;
-                        cell_h(I, e, s, rec + R::H + 9, h0, h1, h2);     // h4 (+ Hm(r+dr))
+                        cell_h(te, s, rec + R::H + 9, h0, h1, h2);     // h4 (+ Hm(r+dr))
                         if constexpr (F64) {
                             Sp<double> a = sp_d(C.p0[k][s]);
                             Sp<double> q1, q3, q4;
@@ -1215,7 +1269,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 q1 = C.k1[k][s]; q3 = C.k3[k][s]; q4 = C.k4[k][s];
                             } else {
                                 double x0, x1, x2;
-                                cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);
+                                cell_h(te, s, rec + R::H + 0, x0, x1, x2);
                                 double lam = lam_d(x0, x1, x2);
                                 PropD P1, P0, Pb, P4;
                                 if constexpr (DBL) {
@@ -1226,10 +1280,10 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 }
                                 q1 = apply_d(P1, a);
                                 q3 = apply_d(P0, a);
-                                cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);
+                                cell_h(te, s, rec + R::H + 3, x0, x1, x2);
                                 prop_d(x0, x1, x2, lam_d(x0, x1, x2), rec[R::DL + 1], Pb);
                                 q1 = sp_avg(apply_d(Pb, a), q1);
-                                cell_h(I, e, s, rec + R::H + 6, x0, x1, x2);
+                                cell_h(te, s, rec + R::H + 6, x0, x1, x2);
                                 prop_d(x0, x1, x2, lam_d(x0, x1, x2), rec[R::DL + 1], P4);
                                 q4 = apply_d(P4, a);
                             }
@@ -1238,7 +1292,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             Sp<double> q3n = apply_d(P2, q3);
                             Sp<double> fin = sp_avg(q3n, q4);
                             dsq = dnorm2<double>(fin, q1);
-                            src_add<CT>(I, e, s, fin, X0, X2, X3);
+                            src_add<CT>(te, s, fin, X0, X2, X3);
                             if (!STREAM) C.k4[k][s] = fin;
                             if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), fin);
                         } else {
@@ -1260,10 +1314,13 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 // B; modes 1/2 recompute them here in the same order.
                                 PropF P, PR, Q;
                                 if (MODE == 0) {
-                                    Q = pf_unpack(C.k1[k][s]); d3 = C.k3[k][s]; d4 = C.k4[k][s];
+                                    const float4 o = s_keep[(k * NS + s) * B + tid];
+                                    Q = PropF{o.x, o.y, o.z, o.w};
+                                    d3 = C.k3[k][s];
+                                    d4 = C.k4[k][s];
                                 } else {
                                 double x0, x1, x2;
-                                cell_h(I, e, s, rec + R::H + 0, x0, x1, x2);                    // h0
+                                cell_h(te, s, rec + R::H + 0, x0, x1, x2);                    // h0
                                 if constexpr (DBL) {
                                     const float f0 = (float)x0, f1 = (float)x1, f2 = (float)x2;
                                     float cm1, sc, scr;
@@ -1277,10 +1334,10 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 d3 = delta_f(P, a);
                                 Q = pf_add(Q, PR);
                                 }
-                                cell_h(I, e, s, rec + R::H + 3, x0, x1, x2);                    // h1
+                                cell_h(te, s, rec + R::H + 3, x0, x1, x2);                    // h1
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
                                 Q = pf_sub(Q, PR);
-                                cell_h(I, e, s, rec + R::H + 6, x0, x1, x2);                    // h3
+                                cell_h(te, s, rec + R::H + 6, x0, x1, x2);                    // h3
                                 prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
                                 d4 = delta_f(P, a);
                                 Q = pf_add(Q, PR);
@@ -1290,7 +1347,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 const Sp<float> u = delta_f(P, d3);
                                 const Sp<float> d3n = sp_add(d3, sp_add(delta_f(P, a), u));
                                 const double* hd = rec + R::HD;
-                                const double hv0 = I.hv[e], hv1 = I.hv[I.ne + e];
+                                const double hv0 = te[0], hv1 = te[1];
                                 const double sg = (s & 1) ? -1.0 : 1.0;
                                 const PropF PD{0.f, (float)-fma(hd[3], hv0, sg * hd[0]), (float)hd[2],
                                                (float)-fma(hd[3], hv1, sg * hd[1])};
@@ -1301,7 +1358,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             }
                             Sp<double> nd = candidate(sp_d(C.p0[k][s]), df, I.renorm);
                             Sp<float> vf = sp_f(nd);
-                            src_add<CT>(I, e, s, vf, X0, X2, X3);
+                            src_add<CT>(te, s, vf, X0, X2, X3);
                             if constexpr (std::is_same<ST, float>::value) {
                                 if (!STREAM) C.k4[k][s] = vf;        // the candidate state itself
                             } else {
@@ -1312,10 +1369,10 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         // error weights of the 4 particles (NBeam.cpp:922): with 2 stored
                         // species nu_x / anti-nu_x share |dpsi|^2 with nu_e / anti-nu_e
                         if (NS == 2) {
-                            s_errb[s * B * CPT + slot] = I.errw[s * ne + e] * dsq;
-                            s_errb[(s + 2) * B * CPT + slot] = I.errw[(s + 2) * ne + e] * dsq;
+                            s_errb[s * B * CPT + slot] = te[2 + s] * dsq;
+                            s_errb[(s + 2) * B * CPT + slot] = te[4 + s] * dsq;
                         } else {
-                            s_errb[s * B * CPT + slot] = I.errw[s * ne + e] * dsq;
+                            s_errb[s * B * CPT + slot] = te[2 + s] * dsq;
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // new psi0 at r+dr
@@ -1354,7 +1411,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
             }
             acc[NMV] = blk_err;
             ok = reduce_publish<NMV, 1>(acc, Mn, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
-            errmax = Mn[NMV];
+            errmax = uni_d(Mn[NMV]);
             if (!ok) break;
         }
 
