@@ -202,11 +202,14 @@ def accuracy_window(prec_name, w=ACC_WINDOW):
             "per_point": pts}
 
 
-def time_short_launches(params, prec_name, psi0, launch, launches=100, per=10):
+def time_short_launches(params, prec_name, psi0, launch, launches=50, per=20):
     """A radial loop cut into short launches (a host that stops every few
     steps to test its stop rules or write output): `launches` eager
-    xf_run_steps calls against one xf_run_graph replay of the same launches,
-    each from the same state, after one untimed pass of each."""
+    xf_run_steps calls of `per` attempts, one xf_run_graph replay of the same
+    launches, and one launch of all launches x per attempts, each from the
+    same state, after one untimed pass of each.  A launch resumes where the
+    previous one stopped (the same attempts, bit for bit), so the three cover
+    the same radii and the difference is the per-call cost."""
     from xflat_amd import capi
     ctx = capi.Context(params, precision=_prec(prec_name))
     ctx.upload(psi0, params.R0, DR0)
@@ -214,19 +217,25 @@ def time_short_launches(params, prec_name, psi0, launch, launches=100, per=10):
         _set_shape(ctx, params, launch)
     cells = params.abins * params.ebins
     out = {"launches": launches, "steps_per_launch": per}
-    for name in ("eager", "graph"):
+    wall = {}
+    for name in ("eager", "graph", "one_launch"):
         for timed in (False, True):
             ctx.upload(psi0, params.R0, DR0)
             t0 = time.perf_counter()
             if name == "eager":
                 steps = sum(ctx.run(per)[0].steps for _ in range(launches))
-            else:
+            elif name == "graph":
                 steps = ctx.run_graph(launches, per).steps
+            else:
+                steps = ctx.run(launches * per)[0].steps
             t1 = time.perf_counter()
+        wall[name] = t1 - t0
         out[f"{name}_value"] = cells * steps / (t1 - t0)
         out[f"{name}_ms_per_step"] = 1e3 * (t1 - t0) / max(1, steps)
         out[f"{name}_radius_km"] = ctx.download()[1]
     ctx.close()
+    out["eager_fraction_of_one_launch"] = wall["one_launch"] / wall["eager"]
+    out["fixed_us_per_call"] = 1e6 * (wall["eager"] - wall["one_launch"]) / launches
     return out
 
 
@@ -498,7 +507,8 @@ def bench_ma(a, prec_name):
     out["short_launches"] = time_short_launches(p, prec_name, psi0, (li["mode"], li["cpt"], li["beams_per_tile"]))
     s = out["short_launches"]
     log(f"{s['launches']} launches x {s['steps_per_launch']} steps: eager {s['eager_value']:.3e}, "
-        f"graph {s['graph_value']:.3e} beam-steps/s")
+        f"graph {s['graph_value']:.3e}, one launch {s['one_launch_value']:.3e} beam-steps/s "
+        f"({s['fixed_us_per_call']:.1f} us per call)")
     if prec_name == "fp32":
         mx = time_ctx(p, "mixed", psi0, a.steps, a.warmup, a.tune_steps)
         out["fp64_state_mixed"] = dict(

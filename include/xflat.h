@@ -221,10 +221,13 @@ int  xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop,
  * XF_NOALLOC_MAX_LOG attempts; otherwise XF_EINVAL.  Every call on a GPU
  * shares that GPU's array: calls that overlap in time on one GPU (two host
  * threads, or two streams) race and give undefined results.  Each call
- * clears what must start at zero (status, rows) before its stepper runs. */
+ * clears what must start at zero (status, rows) before its stepper runs,
+ * and forms psi0's moments there when the context holds none for its
+ * launch shape. */
 #define XF_NOALLOC_MAX_WORKGROUPS 1024
 #define XF_NOALLOC_MAX_LOG        4096
-#define XF_NOALLOC_SCRATCH_BYTES  4330496   /* 1 KiB descriptor + 4 KiB map + 128 KiB log + 4 MiB rows */
+#define XF_NOALLOC_SCRATCH_BYTES  4379648   /* 1 KiB descriptor + 4 KiB map + 128 KiB log + 4 MiB rows
+                                               + 48 KiB moments' partial sums */
 int  xf_run_steps_noalloc(xf_ctx* ctx, int max_steps, const xf_stop* stop,
                           xf_step_rec* log, xf_run_info* info);
 /* The radial loop as a HIP graph: `launches` launches of up to max_steps

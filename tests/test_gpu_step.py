@@ -118,17 +118,31 @@ def test_fused_fp64_four_species(name):
         assert np.abs(psi - g["psi"][-1]).max() <= 1e-7
 
 
-def test_fused_chunked_equals_single_launch():
-    """Splitting the run into launches must not change anything."""
+@pytest.mark.parametrize("prec,shape", [("fp64", "default"), ("fp32", "default"), ("mixed", "default"),
+                                        ("fp32", "mode4"), ("fp64", "mode4")])
+def test_fused_chunked_equals_single_launch(prec, shape):
+    """Splitting the run into launches must not change anything: a launch
+    resumes with the moments and exchange generation the previous one left,
+    so 120 attempts in chunks of 17 (and of 1 at the end) give the step log,
+    state and radius of one launch bit for bit, in every precision, resident
+    and streaming (mode 4, several tiles per workgroup)."""
     capi = _capi()
+    pr = {"fp64": capi.XF_FP64, "mixed": capi.XF_MIXED, "fp32": capi.XF_FP32}[prec]
     g, p = _golden("ma_r50_ih.npz")
-    with capi.Context(p, precision=capi.XF_FP64) as a, capi.Context(p, precision=capi.XF_FP64) as b:
-        a.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
-        b.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+    grid = 8 if shape == "mode4" else 0
+    with capi.Context(p, precision=pr, grid=grid) as a, capi.Context(p, precision=pr, grid=grid) as b:
+        for c in (a, b):
+            if shape == "mode4":
+                c.set_launch(4, 2, 64 * (((2 * p.ebins + 1) // 2 + 63) // 64), 2)
+            c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
         ra = _run_all(a, 120)
-        rb = _run_all(b, 120, chunk=17)
+        rb = np.concatenate([_run_all(b, 110, chunk=17), _run_all(b, 10, chunk=1)])
+        print(f"[chunked] {prec} {shape}: {len(ra)} attempts, r = {a.download()[1]!r}", flush=True)
         assert np.array_equal(ra, rb)
-        assert np.array_equal(a.download()[0], b.download()[0])
+        pa, ra_, da = a.download()
+        pb, rb_, db = b.download()
+        assert ra_ == rb_ and da == db
+        assert np.array_equal(pa, pb)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed", "fp32"])

@@ -58,8 +58,20 @@ struct XfInst {
     float *d_hvf = nullptr, *d_srcwf = nullptr, *d_errwf = nullptr;
     void* d_psi[2] = {nullptr, nullptr};
     XfRunScalars* d_rs = nullptr;
+    // pinned host copy of the run scalars, written by the stepper's workgroup
+    // 0 as a launch ends (h_rs on the host, hd_rs its device address): the
+    // host reads a launch's outcome without a device-to-host copy
+    XfRunScalars* h_rs = nullptr;
+    XfRunScalars* hd_rs = nullptr;
+    XfRunScalars last{};           // the copy as the last launch left it
     unsigned long long* d_rows = nullptr;
     int rows_nblk = 0;
+    // the rows hold tags of this instance's generations below gen0 only (see
+    // XF_MAX_EXCHANGES): the next launch may continue without clearing them
+    bool rows_ok = false;
+    unsigned gen0 = 0;
+    // the launch shape for which the run scalars hold psi0's moments (0: none)
+    long long m0_key = 0;
     XfInstDev dev{};
 
     ~XfInst() {
@@ -67,6 +79,7 @@ struct XfInst {
                       d_psi[0], d_psi[1], d_rs, d_rows};
         for (void* q : ps)
             if (q) (void)hipFree(q);
+        if (h_rs) (void)hipHostFree(h_rs);
     }
 
     long long ncells() const { return (long long)t.nbeam * t.ne; }
@@ -113,6 +126,17 @@ struct XfInst {
         }
         if ((rc = dalloc(&d_rs, 1))) return rc;
         HIPCHK(hipMemset(d_rs, 0, sizeof(XfRunScalars)));
+        {
+            void* h = nullptr;
+            hipError_t e = hipHostMalloc(&h, sizeof(XfRunScalars), hipHostMallocCoherent);
+            if (e != hipSuccess) return xf_fail(XF_ENOMEM, "pinned run scalars: %s", hipGetErrorString(e));
+            h_rs = (XfRunScalars*)h;
+            std::memset(h_rs, 0, sizeof(XfRunScalars));
+            void* hd = nullptr;
+            e = hipHostGetDevicePointer(&hd, h, 0);
+            if (e != hipSuccess) return xf_fail(XF_EHIP, "hipHostGetDevicePointer: %s", hipGetErrorString(e));
+            hd_rs = (XfRunScalars*)hd;
+        }
         fill_dev();
         return XF_OK;
     }
@@ -126,6 +150,7 @@ struct XfInst {
         int rc = dalloc(&d_rows, (size_t)2 * XF_ROW_REPLICAS * XF_ROWW * nblk);
         if (rc) return rc;
         rows_nblk = nblk;
+        rows_ok = false;
         dev.rows = d_rows;
         return XF_OK;
     }
@@ -141,7 +166,7 @@ struct XfInst {
         d.nb0 = p.nb0; d.hNS = p.hNS; d.has_matter = p.has_matter; d.renorm = renorm;
         d.eps0 = p.eps0; d.ksi = p.ksi; d.max_dr = p.max_dr; d.Rn = p.Rn;
         d.psi[0] = d_psi[0]; d.psi[1] = d_psi[1];
-        d.rs = d_rs; d.rows = d_rows;
+        d.rs = d_rs; d.rs_host = hd_rs; d.rows = d_rows;
         d.log = nullptr;
         d.trace = nullptr;
         d.trace_steps = 0;
@@ -185,6 +210,9 @@ struct XfInst {
         XfRunScalars rs{};
         rs.r = r; rs.dr = dr; rs.cur = 0;
         HIPCHK(hipMemcpy(d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
+        rows_ok = false;                 // the generation count restarts with the state
+        m0_key = 0;                      // and the moments are the new state's, not yet formed
+        last = rs;
         return XF_OK;
     }
 
@@ -245,6 +273,8 @@ struct xf_ensemble {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool shaped = false;
+    std::vector<XfInstDev> shadow;      // what d_insts holds (refreshed only when it differs)
+    double* d_part = nullptr;           // moments' partial sums [inst][XF_MOM_PARTS][NM * 3]
     // the radial loop as an instantiated graph (xf_run_graph), kept while its
     // key -- launches, steps per launch, launch shape, descriptor arrays -- holds
     hipGraphExec_t gexec = nullptr;
@@ -258,6 +288,7 @@ struct xf_ensemble {
         if (d_insts) (void)hipFree(d_insts);
         if (d_blk_inst) (void)hipFree(d_blk_inst);
         if (d_log) (void)hipFree(d_log);
+        if (d_part) (void)hipFree(d_part);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -398,6 +429,8 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
         in->dev.tchunk = L.mode == 4 ? std::max(1, L.wbeams / bpts[i]) : 1;
         in->dev.blk0 = b0;
         in->dev.nblk = nblk[i];
+        in->dev.launch_key = ((long long)nblk[i] << 44) | ((long long)L.block << 32) | ((long long)bpts[i] << 24) |
+                             ((long long)(in->dev.tchunk & 0xFFFF) << 8) | ((long long)L.cpt << 4) | L.mode;
         for (int j = 0; j < nblk[i]; ++j) blk_inst[b0 + j] = (int)i;
         b0 += nblk[i];
         devs[i] = in->dev;
@@ -409,6 +442,8 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
     E->d_insts = nullptr;
     E->d_blk_inst = nullptr;
     if ((rc = dput(&E->d_insts, devs)) || (rc = dput(&E->d_blk_inst, blk_inst))) return rc;
+    E->shadow = devs;
+    for (auto& in : E->inst) in->rows_ok = false;       // a new workgroup map
     E->L = L;
     E->shaped = true;
     return XF_OK;
@@ -455,6 +490,62 @@ static int choose_shape(xf_ensemble* E) {
     return xf_fail(XF_EINVAL, "no launch shape fits the device: %s", xf_last_error());
 }
 
+// The descriptor of a one-instance launch on the device, copied only when it
+// changed (stop predicates, the log, a trace).
+static int ens_put_desc(xf_ensemble* E, const XfInstDev& d) {
+    if (E->shadow.size() == 1 && std::memcmp(&d, &E->shadow[0], sizeof d) == 0) return XF_OK;
+    HIPCHK(hipMemcpyAsync(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice, E->stream));
+    E->shadow.assign(1, d);
+    return XF_OK;
+}
+
+// Clears the rows (and status) of every instance unless all of them can
+// continue their generation count for a launch of max_steps steps.
+static int ens_prepare_if_needed(xf_ensemble* E, int max_steps) {
+    bool need = false;
+    for (auto& in : E->inst)
+        if (!in->rows_ok || (long long)in->gen0 + 3ll * max_steps + 2 > XF_MAX_EXCHANGES) need = true;
+    if (!need) return XF_OK;
+    int rc = xf_step_prepare(E->d_insts, (int)E->inst.size(), E->stream);
+    if (rc) return rc;
+    for (auto& in : E->inst) {
+        in->rows_ok = true;
+        in->gen0 = 0;
+    }
+    return XF_OK;
+}
+
+// psi0's moments for every instance unless the run scalars of all of them
+// hold those of the current launch shape (d_insts / d_part: the launch's
+// descriptors and a partial-sum buffer of XF_MOM_PARTS * 12 doubles per
+// instance).
+static int ens_moments_if_needed(xf_ensemble* E, const XfInstDev* d_insts, double* d_part) {
+    bool need = false;
+    for (auto& in : E->inst)
+        if (in->m0_key != in->dev.launch_key) need = true;
+    if (!need) return XF_OK;
+    int nparts = 1;
+    for (auto& in : E->inst) nparts = std::max(nparts, std::min(XF_MOM_PARTS, in->t.nbeam));
+    int rc = xf_moments_launch(E->L, E->geom, d_insts, (int)E->inst.size(), nparts, d_part, E->stream);
+    if (rc) return rc;
+    for (auto& in : E->inst) in->m0_key = in->dev.launch_key;
+    return XF_OK;
+}
+static int ens_part_buffer(xf_ensemble* E) {
+    if (E->d_part) return XF_OK;
+    return dalloc(&E->d_part, E->inst.size() * (size_t)XF_MOM_PARTS * 12);
+}
+
+// After a launch drained: every instance's outcome from its pinned copy.
+static void ens_collect(xf_ensemble* E) {
+    for (auto& in : E->inst) {
+        in->last = *in->h_rs;
+        in->gen0 = in->last.gen0;
+        in->m0_key = in->last.m0_ok ? in->last.m0_key : 0;
+        if (in->last.status) in->rows_ok = false;
+    }
+}
+
 static int ens_launch(xf_ensemble* E, int max_steps, xf_step_rec* log, double* ms) {
     if (!E->shaped) {
         int rc = choose_shape(E);
@@ -468,44 +559,42 @@ static int ens_launch(xf_ensemble* E, int max_steps, xf_step_rec* log, double* m
             if (rc) return rc;
             E->log_cap = max_steps;
         }
+    }
+    if (E->inst.size() == 1) {
         XfInstDev d = E->inst[0]->dev;
-        d.log = E->d_log;
-        HIPCHK(hipMemcpyAsync(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice, E->stream));
-    } else if (E->inst.size() == 1) {
-        XfInstDev d = E->inst[0]->dev;
-        d.log = nullptr;
-        HIPCHK(hipMemcpyAsync(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice, E->stream));
+        d.log = log ? E->d_log : nullptr;
+        int rc = ens_put_desc(E, d);
+        if (rc) return rc;
     }
     if (3ll * max_steps + 1 > XF_MAX_EXCHANGES)
         return xf_fail(XF_EINVAL, "%d steps in one launch exceed the exchange tags", max_steps);
-    int rc0 = xf_step_prepare(E->d_insts, (int)E->inst.size(), E->stream);
+    int rc0 = ens_prepare_if_needed(E, max_steps);
     if (rc0) return rc0;
+    if ((rc0 = ens_part_buffer(E)) || (rc0 = ens_moments_if_needed(E, E->d_insts, E->d_part))) return rc0;
     HIPCHK(hipEventRecord(E->ev0, E->stream));
     int rc = xf_step_launch_geom(E->L, E->geom, E->d_insts, E->d_blk_inst, max_steps, E->stream);
     if (rc) return rc;
     HIPCHK(hipEventRecord(E->ev1, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
+    ens_collect(E);
     float f = 0;
     HIPCHK(hipEventElapsedTime(&f, E->ev0, E->ev1));
     if (ms) *ms = f;
     if (log && E->inst.size() == 1) {
-        XfRunScalars rs;
-        rc = E->inst[0]->scalars(&rs);
-        if (rc) return rc;
-        if (rs.steps_call > 0)
-            HIPCHK(hipMemcpy(log, E->d_log, (size_t)rs.steps_call * sizeof(xf_step_rec), hipMemcpyDeviceToHost));
+        const long long n = E->inst[0]->last.steps_call;
+        if (n > 0) HIPCHK(hipMemcpy(log, E->d_log, (size_t)n * sizeof(xf_step_rec), hipMemcpyDeviceToHost));
     }
     return XF_OK;
 }
 
 // `launches` launches of up to max_steps steps from one instantiated graph:
-// the rows and status are cleared once before it (the graph's own clears keep
-// an error status, so a failed launch ends the rest).  Per-instance run
-// scalars before the graph go to *before.
+// the rows and status are cleared once before it, and the launches continue
+// one generation count (a failed launch leaves its status, which ends the
+// rest).  Per-instance run scalars before the graph go to *before.
 static int ens_run_graph(xf_ensemble* E, int launches, int max_steps, std::vector<XfRunScalars>* before, double* ms) {
     if (launches < 1 || max_steps < 1) return xf_fail(XF_EINVAL, "graph of %d launches x %d steps", launches, max_steps);
-    if (3ll * max_steps + 1 > XF_MAX_EXCHANGES)
-        return xf_fail(XF_EINVAL, "%d steps in one launch exceed the exchange tags", max_steps);
+    if ((long long)launches * (3ll * max_steps + 1) + 1 > XF_MAX_EXCHANGES)
+        return xf_fail(XF_EINVAL, "%d launches of %d steps exceed the exchange tags", launches, max_steps);
     if (!E->shaped) {
         int rc = choose_shape(E);
         if (rc) return rc;
@@ -513,7 +602,8 @@ static int ens_run_graph(xf_ensemble* E, int launches, int max_steps, std::vecto
     if (E->inst.size() == 1) {
         XfInstDev d = E->inst[0]->dev;
         d.log = nullptr;
-        HIPCHK(hipMemcpy(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice));
+        int rc = ens_put_desc(E, d);
+        if (rc) return rc;
     }
     before->resize(E->inst.size());
     for (size_t i = 0; i < E->inst.size(); ++i) {
@@ -535,10 +625,18 @@ static int ens_run_graph(xf_ensemble* E, int launches, int max_steps, std::vecto
     }
     int rc = xf_step_prepare(E->d_insts, (int)E->inst.size(), E->stream);
     if (rc) return rc;
+    for (auto& in : E->inst) {
+        in->rows_ok = true;
+        in->gen0 = 0;
+    }
+    if ((rc = ens_part_buffer(E)) || (rc = ens_moments_if_needed(E, E->d_insts, E->d_part))) return rc;
     HIPCHK(hipEventRecord(E->ev0, E->stream));
     HIPCHK(hipGraphLaunch(E->gexec, E->stream));
     HIPCHK(hipEventRecord(E->ev1, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
+    // the last launch's copy; the graph's own clears restarted the count
+    // before each launch, so the rows agree with its gen0
+    ens_collect(E);
     float f = 0;
     HIPCHK(hipEventElapsedTime(&f, E->ev0, E->ev1));
     if (ms) *ms = f;
@@ -571,10 +669,8 @@ static int ens_create(const xf_params* p, int n, const xf_opts* o, xf_ensemble**
     return XF_OK;
 }
 
-static int fill_info(XfInst* in, xf_run_info* info, double ms) {
-    XfRunScalars rs;
-    int rc = in->scalars(&rs);
-    if (rc) return rc;
+// xf_run_info from run scalars (of the device, or the copy a launch left).
+static int fill_info_from(const XfRunScalars& rs, xf_run_info* info, double ms) {
     if (info) {
         info->steps = rs.steps_call;
         info->accepted = rs.acc_call;
@@ -587,8 +683,21 @@ static int fill_info(XfInst* in, xf_run_info* info, double ms) {
         info->kernel_ms = ms;
         info->stop_reason = rs.status ? 3 : rs.stop_reason;
     }
-    if (rs.status) return xf_fail(rs.status, "stepper status %d (grid barrier did not complete)", rs.status);
+    if (rs.status == XF_EBARRIER && rs.diag_where)
+        return xf_fail(rs.status, "stepper status %d (an exchange wait expired: workgroup %d, %s, waited for tag %u; "
+                                  "%d rows missing, the first, row %d, holds tag %u)", rs.status, rs.diag_wg,
+                       rs.diag_where == 1 ? "poll" : "fetch", rs.diag_tag, rs.diag_missing, rs.diag_row, rs.diag_seen);
+    if (rs.status)
+        return xf_fail(rs.status, "stepper status %d (%s)", rs.status,
+                       rs.status == XF_EBARRIER ? "an exchange wait expired"
+                       : rs.status == XF_ESTATE ? "no moments of psi0 for this launch shape" : "launch error");
     return XF_OK;
+}
+static int fill_info(XfInst* in, xf_run_info* info, double ms) {
+    XfRunScalars rs;
+    int rc = in->scalars(&rs);
+    if (rc) return rc;
+    return fill_info_from(rs, info, ms);
 }
 
 // Times every instantiated (mode, cells per lane, beams per tile) on the
@@ -618,6 +727,9 @@ static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_m
             XfRunScalars rs = sv[i].rs;
             rs.cur = 0;
             HIPCHK(hipMemcpy(in->d_rs, &rs, sizeof rs, hipMemcpyHostToDevice));
+            in->rows_ok = false;     // the trial launches moved the rows past rs.gen0
+            in->m0_key = rs.m0_ok ? rs.m0_key : 0;
+            in->last = rs;
         }
         return XF_OK;
     };
@@ -676,9 +788,11 @@ static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_m
                     XfRunScalars r2;
                     if ((rc = in->scalars(&r2))) break;
                     if (r2.status) {
-                        rc = xf_fail(r2.status, "autotune: shape mode %d, cpt %d, block %d, bpt %d ended with status %d "
-                                                "(%s)", E->L.mode, E->L.cpt, E->L.block, E->L.bpt, r2.status,
-                                     r2.status == XF_EBARRIER ? "an exchange wait expired" : "launch error");
+                        fill_info_from(r2, nullptr, 0.);
+                        const std::string why = xf_last_error();
+                        rc = xf_fail(r2.status, "autotune: shape mode %d, cpt %d, block %d, bpt %d, grid %d ended with "
+                                                "status %d: %s", E->L.mode, E->L.cpt, E->L.block, E->L.bpt, E->L.grid,
+                                     r2.status, why.c_str());
                         break;
                     }
                     done = std::max<long long>(done, r2.steps_call);
@@ -858,7 +972,10 @@ static int inst_set_rn(xf_ensemble* E, size_t i, double rn) {
     }
     in->p.Rn = rn;
     in->dev.Rn = rn;
-    if (E->shaped && E->d_insts) HIPCHK(hipMemcpy(E->d_insts + i, &in->dev, sizeof(XfInstDev), hipMemcpyHostToDevice));
+    if (E->shaped && E->d_insts) {
+        HIPCHK(hipMemcpy(E->d_insts + i, &in->dev, sizeof(XfInstDev), hipMemcpyHostToDevice));
+        if (i < E->shadow.size()) E->shadow[i] = in->dev;
+    }
     return XF_OK;
 }
 
@@ -876,15 +993,10 @@ int xf_run_steps(xf_ctx* ctx, int max_steps, const xf_stop* stop, xf_step_rec* l
     in->dev.dump_rstep = stop ? stop->dump_rstep : 0.;
     in->dev.dump_last_r = stop ? stop->dump_last_r : 0.;
     in->dev.dump_itrstep = stop ? stop->dump_itrstep : 0;
-    if (E->shaped) {
-        // refresh the descriptor copy (stop predicates)
-        XfInstDev d = in->dev;
-        HIPCHK(hipMemcpy(E->d_insts, &d, sizeof d, hipMemcpyHostToDevice));
-    }
     double ms = 0;
-    int rc = ens_launch(E, max_steps, log, &ms);
+    int rc = ens_launch(E, max_steps, log, &ms);     // refreshes the descriptor copy where it changed
     if (rc) return rc;
-    return fill_info(in, info, ms);
+    return fill_info_from(in->last, info, ms);
 }
 
 // xf_run_steps with the descriptor, the rows and the log in the static
@@ -918,20 +1030,23 @@ int xf_run_steps_noalloc(xf_ctx* ctx, int max_steps, const xf_stop* stop, xf_ste
     d.dump_last_r = stop ? stop->dump_last_r : 0.;
     d.dump_itrstep = stop ? stop->dump_itrstep : 0;
     if ((rc = xf_noalloc_setup(d, A, E->L.grid, E->stream))) return rc;
+    // the shared rows start from zero (and the count from 0) on every call;
+    // the instance's own rows no longer follow its count
+    in->rows_ok = false;
     if ((rc = xf_step_prepare(A.desc, 1, E->stream))) return rc;
+    if ((rc = ens_moments_if_needed(E, A.desc, A.part))) return rc;
     HIPCHK(hipEventRecord(E->ev0, E->stream));
     if ((rc = xf_step_launch_geom(E->L, E->geom, A.desc, A.blk, max_steps, E->stream))) return rc;
     HIPCHK(hipEventRecord(E->ev1, E->stream));
     HIPCHK(hipStreamSynchronize(E->stream));
+    in->last = *in->h_rs;
+    in->gen0 = in->last.gen0;
+    in->m0_key = in->last.m0_ok ? in->last.m0_key : 0;
     float f = 0;
     HIPCHK(hipEventElapsedTime(&f, E->ev0, E->ev1));
-    if (log) {
-        XfRunScalars rs;
-        if ((rc = in->scalars(&rs))) return rc;
-        if (rs.steps_call > 0)
-            HIPCHK(hipMemcpy(log, A.log, (size_t)rs.steps_call * sizeof(xf_step_rec), hipMemcpyDeviceToHost));
-    }
-    return fill_info(in, info, f);
+    if (log && in->last.steps_call > 0)
+        HIPCHK(hipMemcpy(log, A.log, (size_t)in->last.steps_call * sizeof(xf_step_rec), hipMemcpyDeviceToHost));
+    return fill_info_from(in->last, info, f);
 }
 
 int xf_run_graph(xf_ctx* ctx, int launches, int max_steps, xf_run_info* info) {
@@ -1090,7 +1205,7 @@ int xf_ensemble_run_steps(xf_ensemble* ens, int max_steps, xf_run_info* info) {
     if (rc) return rc;
     int first_err = XF_OK;
     for (size_t i = 0; i < ens->inst.size(); ++i) {
-        rc = fill_info(ens->inst[i].get(), info ? info + i : nullptr, ms);
+        rc = fill_info_from(ens->inst[i]->last, info ? info + i : nullptr, ms);
         if (rc && !first_err) first_err = rc;
     }
     return first_err;

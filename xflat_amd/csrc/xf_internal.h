@@ -45,6 +45,7 @@ double xf_host_hm(const xf_params* p, const XfTables* t, double r);
 // ---- device descriptor of one instance for the fused stepper --------------
 // Moment layout: per source point NM = 2 (MA: J0, J1) or 4 (MAA: J0, Jx, Jy,
 // Jz) moments times 3 density components (0, 2, 3; component 1 is zero).
+constexpr int XF_M0MAX = 16;                   // moments carried between launches (NM * 3 <= 12)
 struct XfRunScalars {
     double r, dr;
     long long total, accepted;     // counter_tot, counter (Nmr.cpp:123)
@@ -54,7 +55,21 @@ struct XfRunScalars {
     int status;                    // 0 ok, XF_EBARRIER ...
     long long steps_call, acc_call;
     int stop_reason;
-    int pad;
+    unsigned gen0;                 // first exchange generation of the next launch
+    // psi0's angular moments at radius m0_r, left by the last launch for the
+    // next one of the same shape (m0_key): the cached energy sums that
+    // Nmr.cpp:117-121 carries from step to step, so a launch resumes where
+    // the previous one stopped instead of recomputing them
+    int m0_ok, pad;
+    long long m0_key;
+    double m0_r;
+    double m0[XF_M0MAX];
+    // the first exchange wait that expired: where (1 poll, 2 fetch; 0 none),
+    // the workgroup, the tag it waited for, the rows missing and the tag the
+    // first missing row held (a row that never came holds an older tag)
+    int diag_where, diag_wg;
+    unsigned diag_tag, diag_seen;
+    int diag_missing, diag_row;
 };
 
 // Cross-workgroup reduction of one instance (all-gather of tagged rows):
@@ -63,8 +78,12 @@ struct XfRunScalars {
 // carrying the exchange's generation + 1 in its upper 32 bits.  Rows are
 // value-major, rows[parity][word][nblk], so word w of all rows is one
 // contiguous array.  Every workgroup fetches every row and reduces them in
-// the same order.  The buffer is cleared before each launch (generations
-// restart at 0); a launch performs at most XF_MAX_EXCHANGES exchanges.
+// the same order.  Generations continue from launch to launch (the run
+// scalars' gen0): a slot then holds the tag of its current exchange or of the
+// one two before it, never a later one.  The rows are cleared and the count
+// restarted at 0 only where that chain breaks (new rows or workgroup map, a
+// new state, a failed launch, the no-allocation path's own rows) or where the
+// tags would pass XF_MAX_EXCHANGES.
 constexpr int XF_NVMAX = 32;                   // max values per partial row
 constexpr int XF_ROWW = 2 * XF_NVMAX;          // words per row
 constexpr int XF_ROW_REPLICAS = 4;             // copies of every row (reads spread over more lines)
@@ -96,6 +115,7 @@ struct XfInstDev {
     // state and work buffers
     void* psi[2];                  // [ns*4][ncells] planes, double or float
     XfRunScalars* rs;
+    XfRunScalars* rs_host;         // pinned host copy, written by workgroup 0 as a launch ends
     unsigned long long* rows;      // [2][XF_ROW_REPLICAS][XF_ROWW][nblk] tagged partial rows
     xf_step_rec* log;             // optional per-attempt log
     long long* trace;              // optional [steps][16] real-time stamps of workgroup 0
@@ -105,6 +125,11 @@ struct XfInstDev {
     double dump_rstep, dump_last_r;
     int dump_itrstep;
     int tchunk;                    // mode 4: tiles whose beam records LDS holds at once
+    int pad3;
+    // the launch shape (workgroups, block, tile, chunk, cells per lane,
+    // mode), which fixes how the moments' partial sums associate: carried
+    // moments are used by launches of the same shape only
+    long long launch_key;
 };
 
 // ---- launch configuration ---------------------------------------------------
@@ -121,6 +146,16 @@ struct XfLaunch {
     int ne;            // largest energy-bin count of the launch's instances (per-energy LDS table)
 };
 
+constexpr size_t XF_LDS_MAX = 160 * 1024;       // LDS per workgroup (and per CU) on gfx950
+constexpr size_t XF_LDS_GRANULE = 2048;         // LDS per workgroup counted in these units for co-residency
+
+// psi0's moments of every instance of a launch into its run scalars, keyed to
+// the instance's launch shape (xf_moments.hip): nparts slices per instance,
+// partial sums in d_part [ninst][nparts][NM * 3].
+constexpr int XF_MOM_PARTS = 512;
+int xf_moments_launch(const XfLaunch& L, int geom, const XfInstDev* d_insts, int ninst, int nparts, double* d_part,
+                      void* stream);
+
 // Launch helpers of the fused stepper (xf_step_host.hip).
 bool xf_step_has_shape(const XfLaunch& L, int geom);
 size_t xf_step_smem_bytes(const XfLaunch& L, int geom);
@@ -130,7 +165,8 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
                         void* stream);
 int xf_step_func_info(const XfLaunch& L, int geom, int* vgprs, int* spill_bytes);
 // Clears the run status and the exchange rows of every instance of a launch
-// (a kernel, so the launch sequence holds no memset node).
+// and restarts their generation count (a kernel, so the launch sequence holds
+// no memset node).
 int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream);
 // The no-allocation entry point's device memory (xf_noalloc.hip): the
 // regions of the static scratch array on the current device.
@@ -139,6 +175,7 @@ struct XfNoAllocArea {
     int* blk;                      // workgroup -> instance map (all 0)
     xf_step_rec* log;              // per-attempt log
     unsigned long long* rows;      // exchange rows for up to XF_NOALLOC_MAX_WORKGROUPS
+    double* part;                  // moments' partial sums (XF_MOM_PARTS x 12)
 };
 int xf_noalloc_area(XfNoAllocArea* out);
 // Writes descriptor `d` into the area and clears the map for `grid`

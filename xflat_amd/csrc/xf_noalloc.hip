@@ -1,8 +1,9 @@
 // xf_noalloc.hip -- device memory of the stepper entry point that allocates
 // nothing (xf_run_steps_noalloc): the launch descriptor, the workgroup ->
-// instance map, the per-attempt log and the cross-workgroup exchange rows
-// live in one file-scope device array, `scratch`, loaded with the code
-// object (one copy per GPU, held until the process ends).
+// instance map, the per-attempt log, the cross-workgroup exchange rows and
+// the partial sums of psi0's moments live in one file-scope device array,
+// `scratch`, loaded with the code object (one copy per GPU, held until the
+// process ends).
 //
 // Host code never passes `scratch` itself to a kernel or a copy: in host
 // code that name is a host-side stand-in, not a device address.  The array's
@@ -27,9 +28,11 @@ constexpr size_t NA_LOG_BYTES = sizeof(xf_step_rec) * XF_NOALLOC_MAX_LOG;
 constexpr size_t NA_ROWS = NA_LOG + NA_LOG_BYTES;                        // exchange rows
 constexpr size_t NA_ROWS_BYTES =
     sizeof(unsigned long long) * 2 * XF_ROW_REPLICAS * XF_ROWW * (size_t)XF_NOALLOC_MAX_WORKGROUPS;
-constexpr size_t NA_TOTAL = NA_ROWS + NA_ROWS_BYTES;
+constexpr size_t NA_PART = NA_ROWS + NA_ROWS_BYTES;                      // moments' partial sums
+constexpr size_t NA_PART_BYTES = sizeof(double) * XF_MOM_PARTS * 12;
+constexpr size_t NA_TOTAL = NA_PART + NA_PART_BYTES;
 static_assert(sizeof(XfInstDev) <= NA_DESC_BYTES, "descriptor slot");
-static_assert(NA_BLK % 256 == 0 && NA_LOG % 256 == 0 && NA_ROWS % 256 == 0, "alignment");
+static_assert(NA_BLK % 256 == 0 && NA_LOG % 256 == 0 && NA_ROWS % 256 == 0 && NA_PART % 256 == 0, "alignment");
 static_assert(NA_TOTAL == XF_NOALLOC_SCRATCH_BYTES, "the size stated in the header");
 
 constexpr int NA_MAX_DEVICES = 64;
@@ -68,6 +71,7 @@ int xf_noalloc_area(XfNoAllocArea* out) {
     out->blk = (int*)(base + NA_BLK);
     out->log = (xf_step_rec*)(base + NA_LOG);
     out->rows = (unsigned long long*)(base + NA_ROWS);
+    out->part = (double*)(base + NA_PART);
     return XF_OK;
 }
 

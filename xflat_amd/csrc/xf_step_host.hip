@@ -4,7 +4,6 @@
 #include <algorithm>
 #include "xf_internal.h"
 
-constexpr size_t XF_LDS_MAX = 160 * 1024;        // LDS per workgroup on gfx950
 
 using XfStepFn = void (*)(const XfInstDev*, const int*, int);
 // instantiation tables (xf_k_*.hip): workgroups of up to 512 threads, and
@@ -109,6 +108,16 @@ int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {
     int per = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)f, L.block, smem);
     if (e != hipSuccess) return xf_fail(XF_EHIP, "occupancy query failed: %s", hipGetErrorString(e));
+    // The runtime's figure can overstate what LDS holds: a grid of 1-wave
+    // workgroups of 26,904 B (dynamic + static) each, six per CU by that
+    // figure, started five per CU (the sixth never became resident and the
+    // others' exchange waits expired).  Workgroups per CU are also capped by
+    // LDS counted in XF_LDS_GRANULE units, static LDS included.
+    hipFuncAttributes a;
+    e = hipFuncGetAttributes(&a, (const void*)f);
+    if (e != hipSuccess) return xf_fail(XF_EHIP, "hipFuncGetAttributes: %s", hipGetErrorString(e));
+    const size_t per_wg = (smem + a.sharedSizeBytes + XF_LDS_GRANULE - 1) / XF_LDS_GRANULE * XF_LDS_GRANULE;
+    if (per_wg > 0) per = std::min<int>(per, (int)(XF_LDS_MAX / per_wg));
     *max_blocks = per * ncu;
     return XF_OK;
 }
@@ -150,13 +159,16 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
     return XF_OK;
 }
 
-// Status and exchange rows of every instance to zero (generation tags start
-// at 1, so a cleared word matches no exchange).  keep_status leaves a nonzero
-// status in place: the stepper then exits at once, so an error in one launch
-// of a graph ends the later ones and is reported after the graph.
-__global__ void k_step_prepare(const XfInstDev* __restrict__ insts, int keep_status) {
+// Status and exchange rows of every instance to zero and the generation
+// count restarted (generation tags start at 1, so a cleared word matches no
+// exchange).
+__global__ void k_step_prepare(const XfInstDev* __restrict__ insts) {
     const XfInstDev& I = insts[blockIdx.y];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !keep_status) I.rs->status = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        I.rs->status = 0;
+        I.rs->gen0 = 0u;
+        I.rs->diag_where = 0;
+    }
     const long long n = 2ll * XF_ROW_REPLICAS * XF_ROWW * I.nblk;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         I.rows[i] = 0ull;
@@ -165,17 +177,19 @@ __global__ void k_step_prepare(const XfInstDev* __restrict__ insts, int keep_sta
 int xf_step_prepare(const XfInstDev* d_insts, int ninst, void* stream) {
     if (ninst <= 0 || ninst > 65535) return xf_fail(XF_EINVAL, "%d instances in one launch", ninst);
     (void)hipGetLastError();    // a launch error, not one left by an earlier handled call
-    hipLaunchKernelGGL(k_step_prepare, dim3(8, ninst), dim3(256), 0, (hipStream_t)stream, d_insts, 0);
+    hipLaunchKernelGGL(k_step_prepare, dim3(8, ninst), dim3(256), 0, (hipStream_t)stream, d_insts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return xf_fail(XF_EHIP, "prepare launch: %s", hipGetErrorString(e));
     return XF_OK;
 }
 
-// The radial loop as one graph: the stepper, then launches - 1 pairs (rows
-// cleared with the status kept, stepper), each node after the previous one.
-// The stepper nodes are ordinary kernel nodes, as the direct launch is (the
-// planner sized the grid to be co-resident).  No memset node: the
-// rows are cleared by a kernel.
+// The radial loop as one graph: `launches` stepper nodes, each after the
+// previous one.  The generation count continues from node to node (each
+// launch leaves it in the run scalars), so the rows need no clear between
+// them; a nonzero status left by one launch makes the later ones exit at once,
+// and the error is reported after the graph.  The stepper nodes are ordinary
+// kernel nodes, as the direct launch is (the planner sized the grid to be
+// co-resident).
 int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, const int* d_blk_inst, int ninst,
                         int max_steps, int launches, hipGraphExec_t* exec) {
     XfStepFn f = pick(L, geom);
@@ -191,29 +205,16 @@ int xf_step_build_graph(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
     hipGraph_t g = nullptr;
     hipError_t e = hipGraphCreate(&g, 0);
     if (e != hipSuccess) return xf_fail(XF_EHIP, "hipGraphCreate: %s", hipGetErrorString(e));
-    int keep = 1;
     void* step_args[] = {(void*)&d_insts, (void*)&d_blk_inst, (void*)&max_steps};
-    void* prep_args[] = {(void*)&d_insts, (void*)&keep};
     hipKernelNodeParams sp{};
     sp.func = (void*)f;
     sp.gridDim = dim3(L.grid);
     sp.blockDim = dim3(L.block);
     sp.sharedMemBytes = (unsigned)smem;
     sp.kernelParams = step_args;
-    hipKernelNodeParams pp{};
-    pp.func = (void*)k_step_prepare;
-    pp.gridDim = dim3(8, ninst);
-    pp.blockDim = dim3(256);
-    pp.sharedMemBytes = 0;
-    pp.kernelParams = prep_args;
     hipGraphNode_t prev = nullptr;
     for (int k = 0; k < launches && e == hipSuccess; ++k) {
         hipGraphNode_t node = nullptr;
-        if (k > 0) {
-            e = hipGraphAddKernelNode(&node, g, &prev, 1, &pp);
-            if (e != hipSuccess) break;
-            prev = node;
-        }
         e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &sp);
         prev = node;
     }

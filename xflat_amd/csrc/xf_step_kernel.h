@@ -273,15 +273,58 @@ __device__ __forceinline__ void st_word(unsigned long long* p, unsigned long lon
     __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Bounded-wait bookkeeping: 2 s of the 100 MHz real-time counter, or an
-// error flagged by another workgroup.  Returns true when the wait must end.
-__device__ __forceinline__ bool wait_expired(uint64_t t0, int spins, int* status) {
-    if ((spins & 15) != 0) return false;
-    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+// error flagged by another workgroup.  Returns 0 while the wait may go on,
+// 1 when this wait expired (the status is set), 2 when another workgroup's
+// error ends it.
+__device__ __forceinline__ int wait_expired(uint64_t t0, int spins, int* status) {
+    if ((spins & 15) != 0) return 0;
+    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return 2;
     if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
         __hip_atomic_store(status, (int)XF_EBARRIER, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return true;
+        return 1;
     }
-    return false;
+    return 0;
+}
+// After this wave's own wait expired: what it saw in the word array w of the
+// n rows it waited on (the rows whose tag is not `tag`, and the tag the first
+// of them holds), kept by the first workgroup to get here.  One full wave.
+__device__ __noinline__ void note_expiry(XfRunScalars* rs, int where, int lb, unsigned tag,
+                                         const unsigned long long* w, int n) {
+    const int lane = threadIdx.x & 63;
+    int missing = 0, first = n;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        const bool bad = j < n && (unsigned)(ld_word(w + min(j, n - 1)) >> 32) != tag;
+        const unsigned long long m = __ballot(bad);
+        missing += __popcll(m);
+        if (m && first == n) first = j0 + __ffsll((long long)m) - 1;
+    }
+    const unsigned seen = first < n ? (unsigned)(ld_word(w + first) >> 32) : tag;
+    if (lane == 0) {
+        int expect = 0;
+        if (__hip_atomic_compare_exchange_strong(&rs->diag_where, &expect, where, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+            rs->diag_wg = lb;
+            rs->diag_tag = tag;
+            rs->diag_seen = seen;
+            rs->diag_missing = missing;
+            rs->diag_row = first;
+        }
+    }
+}
+
+// The run scalars into the pinned host copy, one 64-bit word at a time (one
+// thread; a struct temporary would live in scratch memory).
+static_assert(sizeof(XfRunScalars) % 8 == 0, "run scalars: whole 64-bit words");
+__device__ __forceinline__ void mirror_scalars(const XfRunScalars* src, XfRunScalars* dst) {
+    constexpr int NW = (int)(sizeof(XfRunScalars) / 8);
+    const unsigned long long* s = reinterpret_cast<const unsigned long long*>(src);
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(dst);
+    unsigned long long w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = s[i];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) d[i] = w[i];
 }
 
 // matt::getHm (Matt.cpp:31-64)
@@ -688,7 +731,11 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
             int mine = 1;
             for (int j0 = 0; j0 < nblk; j0 += 256) mine &= rows_current(rows + j0, lane, min(nblk - j0, 256), tag);
             if (__all(mine)) break;
-            if (wait_expired(t0, spins, status)) { fail = 1; break; }
+            if (const int x = wait_expired(t0, spins, status)) {
+                if (x == 1) note_expiry(I.rs, 1, lb, tag, rows, nblk);
+                fail = 1;
+                break;
+            }
         }
         fail = __any(fail);
         if (tid == 0) s_flag[0] = fail;
@@ -741,7 +788,11 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
                     }
                     break;
                 }
-                if (wait_expired(t1, spins, status)) { wfail = 1; break; }
+                if (const int x = wait_expired(t1, spins, status)) {
+                    if (x == 1) note_expiry(I.rs, 2, lb, tag, rows + (size_t)(2 * v0) * nblk, nblk);
+                    wfail = 1;
+                    break;
+                }
             }
         }
         x[0] = (v0 < NV) ? wave_red_dpp<false>(x[0]) : wave_red_dpp<true>(x[0]);
@@ -805,7 +856,14 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     XfRunScalars* rs = I.rs;
     // an error left by an earlier launch of the same graph: nothing to do
     // (read before any workgroup of this launch can change it)
-    if (__hip_atomic_load(&rs->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    if (__hip_atomic_load(&rs->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        if (lb == 0 && tid == 0 && I.rs_host) {
+            mirror_scalars(rs, I.rs_host);
+            I.rs_host->steps_call = 0;
+            I.rs_host->acc_call = 0;
+        }
+        return;
+    }
 
     extern __shared__ double smem[];
     double* s_beam = smem;                                   // [bpt][R::SIZE]
@@ -853,7 +911,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     int cur = __builtin_amdgcn_readfirstlane(rs->cur);
     double last_err = uni_d(rs->last_err);
     double last_dr = uni_d(rs->last_dr);
-    unsigned gen = 0u;                    // generation of the instance barrier
+    unsigned gen = (unsigned)__builtin_amdgcn_readfirstlane((int)rs->gen0);   // generation of the exchanges
     int* status = &rs->status;
 
     Cells<ST, CT, NS, CPT> C;
@@ -956,45 +1014,26 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         load_p0(my_tile0, cur);
     }
 
-    bool ok = true;
-    // ---- moments of psi0 at r (the cached energy sums of Nmr.cpp:117-121)
-    {
-        double acc[XF_NVMAX];
-#pragma unroll
-        for (int v = 0; v < NMV; ++v) acc[v] = 0.;
-        double dr2 = .5 * dr, rdr = r + dr, rdr2 = r + dr2;
-        for (int t = (STREAM ? lb : my_tile0); t < (STREAM ? ntiles : my_tile0 + 1); t += (STREAM ? nblk : 1)) {
-            int nbt = STREAM ? tile_cells(t) : nbt_res;
-            const int jt = TABLE ? (t - lb) / nblk : 0;           // local tile index
-            const double* tb = TABLE ? s_beam + (jt % tch) * bpt * R::SIZE : s_beam;
-            if constexpr (TABLE) {
-                if (jt % tch == 0) {                  // a chunk starts: its records
-                    if (jt > 0) __syncthreads();
-                    table_points(jt, r, rdr2, rdr);
-                    __syncthreads();
-                }
-                load_p0(t, cur);
-            } else {
-                __syncthreads();
-                if (MODE == 2) load_p0(t, cur);      // in flight during the set-up
-                tile_points(t, nbt, r, rdr2, rdr);
-                __syncthreads();
-            }
-#pragma unroll
-            for (int k = 0; k < CPT; ++k) {
-                if (C.bl[k] < 0) continue;
-                const double* rec = tb + C.bl[k] * R::SIZE;
-                CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    if constexpr (F64) src_add<CT>(s_tab + C.e[k] * TS, s, sp_d(C.p0[k][s]), X0, X2, X3);
-                    else src_add<CT>(s_tab + C.e[k] * TS, s, sp_f(C.p0[k][s]), X0, X2, X3);
-                }
-                acc_moments<NM>(acc, rec + R::W + 0 * NM, (double)X0, (double)X2, (double)X3);
+    // ---- moments of psi0 at r (the cached energy sums of Nmr.cpp:117-121):
+    // left by the previous launch of this shape, or by the moments kernels
+    // (xf_moments.hip) the host runs before a launch that has none.  Every
+    // workgroup reads the same run scalars, so all of them leave together
+    // here when the host did not.
+    if (__builtin_amdgcn_readfirstlane(rs->m0_ok) == 0 || uni_ll(rs->m0_key) != I.launch_key || uni_d(rs->m0_r) != r) {
+        if (lb == 0 && tid == 0) {
+            __hip_atomic_store(status, (int)XF_ESTATE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            rs->steps_call = 0;
+            rs->acc_call = 0;
+            if (I.rs_host) {
+                mirror_scalars(rs, I.rs_host);
+                I.rs_host->status = (int)XF_ESTATE;
             }
         }
-        ok = reduce_publish<NMV, 0>(acc, M0, s_errb, xcap, I, lb, gen, status, s_flag);
+        return;
     }
+    if (tid < NMV) M0[tid] = rs->m0[tid];
+    __syncthreads();
+    bool ok = true;
 
     long long steps_call = 0, acc_call = 0;
     int stop_reason = 0;
@@ -1487,6 +1526,20 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         rs->steps_call = steps_call;
         rs->acc_call = acc_call;
         rs->stop_reason = stop_reason;
+        rs->gen0 = gen;
+        rs->m0_ok = ok ? 1 : 0;
+        rs->m0_key = I.launch_key;
+        rs->m0_r = r;
+#pragma unroll
+        for (int v = 0; v < NMV; ++v) rs->m0[v] = M0[v];
+        if (I.rs_host) {
+            // the host reads this copy after the stream drained; a failed
+            // exchange anywhere also fails this workgroup's (it waits for
+            // every row), so its view of the status is the launch's
+            mirror_scalars(rs, I.rs_host);
+            const int st = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            I.rs_host->status = (!ok && st == 0) ? (int)XF_EBARRIER : st;
+        }
     }
 }
 
