@@ -109,8 +109,9 @@ def _state_bytes(prec_name):
 
 
 def _set_shape(ctx, params, launch):
+    from xflat_amd import capi
     mode, cpt, bpt = launch
-    ctx.set_launch(mode, cpt, 64 * (((bpt * params.ebins + cpt - 1) // cpt + 63) // 64), bpt)
+    ctx.set_launch(mode, cpt, capi.block_for(bpt, params.ebins, cpt), bpt)
 
 
 def time_ctx(params, prec_name, psi0, steps, warmup, tune_steps, launch=None, r0=None):

@@ -81,7 +81,9 @@ typedef struct xf_params {
     char   file_prefix[256];
 } xf_params;
 
-/* Launch/precision options of the fused stepper. Zero = automatic. */
+/* Launch/precision options of the fused stepper.  species, block, cpt and
+ * grid: 0 = automatic; renorm and resident: -1 = automatic (0 turns
+ * renormalisation off, resp. forces the streaming modes). */
 typedef struct xf_opts {
     int precision;        /* XF_FP64 / XF_MIXED / XF_FP32                       */
     int species;          /* 0 auto, 2 = SU(2)-partner reduced, 4 = explicit    */
@@ -92,6 +94,9 @@ typedef struct xf_opts {
     int resident;         /* 1 keep state in registers across steps, 0 stream,
                              -1 auto                                            */
     int grid;             /* workgroups (streaming mode), 0 auto                */
+    int mode;             /* output only (xf_get_launch, xf_autotune): the
+                             launch mode in use, 0 keep / 1 recompute (both
+                             resident = 1), 2 / 4 stream (resident = 0)      */
 } xf_opts;
 
 /* One attempted radial step of the fused stepper (Nmr.cpp:153-249). */

@@ -133,7 +133,7 @@ def test_fused_chunked_equals_single_launch(prec, shape):
     with capi.Context(p, precision=pr, grid=grid) as a, capi.Context(p, precision=pr, grid=grid) as b:
         for c in (a, b):
             if shape == "mode4":
-                c.set_launch(4, 2, 64 * (((2 * p.ebins + 1) // 2 + 63) // 64), 2)
+                c.set_launch(4, 2, capi.block_for(2, p.ebins, 2), 2)
             c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
         ra = _run_all(a, 120)
         rb = np.concatenate([_run_all(b, 110, chunk=17), _run_all(b, 10, chunk=1)])
@@ -154,9 +154,13 @@ def test_fused_modes_agree(prec):
     phases over 200 steps), and the resident-keep launch forms the error
     remainders in passes A/B (the compiler may contract them into FMAs
     differently): FP64 takes the same accepted steps and keeps P_ee to the
-    parity bar; the FP32-increment steppers resolve err to ~u/x relative
-    (x = lambda dl), so a decision within that of eps0 can flip: >= 95 % of the
-    decisions agree and P_ee stays within 1e-5."""
+    parity bar.  The FP32-increment steppers' error estimate scatters by a few
+    per cent around FP64's on this grid at eps0 = 1e-8 (the FP32 rounding of
+    the remainder coefficients of grazing beams; pinned in
+    test_fp32_error_estimate_from_identical_states), and their states differ
+    by FP32 rounding from the first step, so near-threshold decisions differ
+    and then the dr sequences: the launches advance the same radius to within
+    5 %, accept as many steps to within 5 % and keep P_ee within 1e-5."""
     capi = _capi()
     pr = {"fp64": capi.XF_FP64, "mixed": capi.XF_MIXED, "fp32": capi.XF_FP32}[prec]
     g, p = _golden("ma_r10_ih.npz")
@@ -165,7 +169,7 @@ def test_fused_modes_agree(prec):
     for mode, cpt in shapes:
         with capi.Context(p, precision=pr) as ctx:
             ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
-            ctx.set_launch(mode, cpt, 64 * ((8 * p.ebins // cpt + 63) // 64), 8)
+            ctx.set_launch(mode, cpt, capi.block_for(8, p.ebins, cpt), 8)
             rec = _run_all(ctx, 200)
             out[(mode, cpt)] = (rec, ctx.download()[0])
     a = out[(1, 1)]
@@ -182,8 +186,55 @@ def test_fused_modes_agree(prec):
         elif prec == "fp64":
             assert dpee <= PEE_TOL and np.abs(a[1] - b[1]).max() <= 1e-7, key
         else:
-            assert np.mean(a[0]["accepted"] == b[0]["accepted"]) >= 0.95, key
+            r0 = float(g["r0"])
+            adv = (b[0]["r"][-1] - r0) / (a[0]["r"][-1] - r0)
+            acc = b[0]["accepted"].sum() / a[0]["accepted"].sum()
+            assert abs(adv - 1.0) < 0.05 and abs(acc - 1.0) < 0.05, (key, adv, acc)
             assert dpee <= 1e-5, key
+
+
+@pytest.mark.parametrize("name,rms,worst", [("ma_r10_ih.npz", 0.08, 0.2), ("ma_r50_ih.npz", 0.01, 0.03)])
+def test_fp32_error_estimate_from_identical_states(name, rms, worst):
+    """The compensated error estimate of the FP32-increment steppers against
+    the FP64 stepper's from the same states, r and dr: 30 states along an
+    FP64 trajectory, one attempted step each.  The FP32 estimate scatters
+    around FP64's by the FP32 rounding of the second-order remainder
+    coefficients, 6e-8 x^2 for x = lambda dl, which is largest for grazing
+    beams (long dl) near the neutrinosphere: a few per cent (absolute ~4e-10)
+    at R = 10 km, eps0 = 1e-8 on the 32 x 16 grid, a few per mille at R = 50
+    km, eps0 = 1e-9.  The mean deviation stays below 2 %, the RMS and the
+    largest below the per-grid bounds; every launch shape of one stepper
+    gives the same estimate to 1e-3."""
+    capi = _capi()
+    g, p = _golden(name)
+    print(f"[err estimate] {name}", flush=True)
+    states = []
+    with capi.Context(p, precision=capi.XF_FP64) as c:
+        c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        for _ in range(30):
+            c.run(7)
+            states.append(c.download())
+
+    def one(prc, shape, st):
+        with capi.Context(p, precision=prc) as c:
+            c.upload(*st)
+            if shape:
+                mode, cpt = shape
+                c.set_launch(mode, cpt, capi.block_for(8, p.ebins, cpt), 8)
+            return c.run(1, log=True)[1]["err"][0]
+
+    e64 = np.array([one(capi.XF_FP64, None, st) for st in states])
+    for prc in (capi.XF_MIXED, capi.XF_FP32):
+        ref = None
+        for shape in ((1, 1), (0, 1), (1, 4), (4, 2)):
+            e = np.array([one(prc, shape, st) for st in states])
+            rel = e / e64 - 1
+            print(f"  precision {prc} {shape}: mean {np.mean(rel):+.2e}, rms {np.sqrt(np.mean(rel ** 2)):.2e}, "
+                  f"max {np.abs(rel).max():.2e}", flush=True)
+            assert abs(np.mean(rel)) < 0.02 and np.sqrt(np.mean(rel ** 2)) < rms and np.abs(rel).max() < worst, (prc, shape)
+            if ref is None:
+                ref = e
+            assert np.abs(e / ref - 1).max() < 1e-3, (prc, shape)
 
 
 @pytest.mark.parametrize("name", ["ma_r50_ih.npz", "sa_r50_nh.npz"])
@@ -514,7 +565,7 @@ def test_graph_replays_equal_eager_launches(prec, shape):
     with capi.Context(p, precision=pr, grid=grid) as cg, capi.Context(p, precision=pr, grid=grid) as ce:
         for c in (cg, ce):
             if shape == "mode4":
-                c.set_launch(4, 2, 64 * (((2 * p.ebins + 1) // 2 + 63) // 64), 2)
+                c.set_launch(4, 2, capi.block_for(2, p.ebins, 2), 2)
             c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
         for rep in range(3):
             info = cg.run_graph(4, 25)
@@ -584,7 +635,7 @@ def test_streaming_table_mode_equals_streaming(prec):
             for mode in (2, 4):
                 with capi.Context(p, precision=pr, grid=grid) as ctx:
                     ctx.init_pure()
-                    ctx.set_launch(mode, cpt, 64 * (((bpt * p.ebins + cpt - 1) // cpt + 63) // 64), bpt)
+                    ctx.set_launch(mode, cpt, capi.block_for(bpt, p.ebins, cpt), bpt)
                     info, _ = ctx.run(120)
                     res[mode] = (info.steps, info.accepted, *ctx.download(), ctx.launch()["grid"])
             a, b = res[2], res[4]

@@ -13,6 +13,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from conftest import ROOT
 
@@ -87,3 +88,109 @@ def test_remainder_coefficient_relative_accuracy():
         exact = dl * float(t) * _q_exact(float(t))
         assert x < 1.0
         assert abs(scr - exact) <= 3e-7 * abs(exact), (lam, dl, scr, exact)
+
+
+# ---- the FP32 wave-uniform small-angle tier and the FP64 series tiers ----
+from fractions import Fraction as Fr  # noqa: E402
+
+
+def _block(src, start, end):
+    i = src.index(start)
+    return src[i:src.index(end, i)]
+
+
+def test_fp32_linear_tier_within_its_stated_bound():
+    """When every lane of the wave has t = x^2 < 2^-10, prop_f2s takes the
+    linear truncations (cos x - 1)/t = -1/2 + t/24 and (sin x/x - 1)/t =
+    -1/6 + t/120: their omitted t^2/720 and t^2/5040 stay below 2.7e-9 and
+    1.2e-9 relative, under FP32 rounding (evaluated as the kernel does, one
+    FMA rounded once)."""
+    src = open(SRC).read()
+    blk = _block(src, "if (__all(x2 < 0x1p-10f))", "} else if (x2 < 1.f)")
+    m = re.findall(r"const float (pc|ps) = fmaf\(x2, " + NUM + ", " + NUM + r"\);", blk)
+    assert [x[0] for x in m] == ["pc", "ps"], m
+    coef = {k: (np.float32(float(c0)), np.float32(float(c1))) for k, c1, c0 in m}
+    assert coef["pc"][0] == np.float32(-0.5) and coef["ps"][0] == np.float32(-1.0 / 6.0)
+    assert coef["pc"][1] == np.float32(1.0 / 24.0) and coef["ps"][1] == np.float32(1.0 / 120.0)
+    ts = np.geomspace(1e-12, 2.0 ** -10, 300).astype(np.float32)
+    # FP32 rounding bounds as for the cubics: -1/6 itself is 3e-8 off in FP32
+    for k, exact, trunc, rnd in (("pc", _p_exact, 2.7e-9, 6e-8), ("ps", _q_exact, 1.2e-9, 1e-7)):
+        c0, c1 = coef[k]
+        lin = (lambda t: -0.5 + t / 24) if k == "pc" else (lambda t: -1 / 6 + t / 120)
+        worst_trunc = max(abs(lin(float(t)) - exact(float(t))) / abs(exact(float(t))) for t in ts)
+        assert worst_trunc < trunc, (k, worst_trunc)
+        worst = max(abs(float(_fma32(t, c1, c0)) - exact(float(t))) / abs(exact(float(t))) for t in ts)
+        assert worst < rnd + trunc, (k, worst)
+
+
+def _fma64(a, b, c):
+    # a*b + c rounded once to FP64 (exact rational arithmetic)
+    return float(Fr(a) * Fr(b) + Fr(c))
+
+
+def _chains(blk):
+    """The two Horner chains (sn, cs) of one tier of prop_ds, as written:
+    'double X = C;' then 'X = fma(u, [-]X, C);' with C = [-]1.0 / N or 0.5."""
+    const = r"(-?1\.0 / [0-9.]+|0\.5)"
+    out = {"sn": [], "cs": []}
+    for m in re.finditer(r"double (sn|cs) = " + const + r";|(sn|cs) = fma\(u, (-?)(sn|cs), " + const + r"\)\s*;", blk):
+        if m.group(1):
+            out[m.group(1)].append(("init", m.group(2)))
+        else:
+            assert m.group(3) == m.group(5)
+            out[m.group(3)].append((m.group(4), m.group(6)))
+
+    def val(c):
+        if c == "0.5":
+            return Fr(1, 2)
+        sign = -1 if c.startswith("-") else 1
+        return sign * Fr(1) / Fr(c.split("/")[1].strip())
+    return {k: [(op, val(c)) for op, c in v] for k, v in out.items()}
+
+
+def _eval_chain(ch, u, exact):
+    """exact=True: rational arithmetic (the truncated series itself);
+    False: every FMA rounded once to FP64, constants rounded to FP64 as
+    the compiler folds 1.0 / N."""
+    x = None
+    for op, c in ch:
+        cc = c if exact else Fr(float(c))
+        if op == "init":
+            x = cc
+        else:
+            xx = -x if op == "-" else x
+            x = (Fr(u) * xx + cc) if exact else Fr(_fma64(float(u), float(xx), float(cc)))
+    return x
+
+
+def _cos_sinc_exact(u, terms=30):
+    # cos x and sin(x)/x for x^2 = u, exact rationals to far below FP64
+    c = sum(Fr((-1) ** k) * Fr(u) ** k / math.factorial(2 * k) for k in range(terms))
+    s = sum(Fr((-1) ** k) * Fr(u) ** k / math.factorial(2 * k + 1) for k in range(terms))
+    return c, s
+
+
+@pytest.mark.parametrize("tier,umax,trunc", [(0, 2.0 ** -10, 2.5e-22), (1, 0.0625, 1e-23)])
+def test_fp64_series_tiers(tier, umax, trunc):
+    """prop_ds below u = x^2 = 2^-10 (to x^8 / x^9) and below 1/16 (to x^16 /
+    x^17): the truncated series differ from cos x and sin(x)/x by less than
+    the stated remainders, and the FP64 evaluation (FMA Horner, as written)
+    is within 2 ulp of them."""
+    src = open(SRC).read()
+    body = _block(src, "__device__ __forceinline__ void prop_ds(", "\n}\n")
+    blocks = [_block(body, "if (u < 0x1p-10) {", "} else if (u < 0.0625)"),
+              _block(body, "} else if (u < 0.0625) {", "} else {")]
+    ch = _chains(blocks[tier])
+    assert len(ch["sn"]) == (4 if tier == 0 else 8) and len(ch["cs"]) == (4 if tier == 0 else 8), ch
+    assert "c = fma(-u, cs, 1.0);" in blocks[tier] and "sc = dl * fma(-u, sn, 1.0);" in blocks[tier]
+    for u in [umax * f for f in (1e-6, 1e-3, 0.1, 0.5, 0.999)]:
+        u = float(u)
+        ce, se = _cos_sinc_exact(u)
+        cs_x, sn_x = _eval_chain(ch["cs"], u, True), _eval_chain(ch["sn"], u, True)
+        c_poly, s_poly = 1 - Fr(u) * cs_x, 1 - Fr(u) * sn_x
+        assert abs(float(c_poly - ce)) < trunc and abs(float(s_poly - se)) < trunc, (u, float(c_poly - ce))
+        cs_f, sn_f = _eval_chain(ch["cs"], u, False), _eval_chain(ch["sn"], u, False)
+        c64 = _fma64(-u, float(cs_f), 1.0)
+        s64 = _fma64(-u, float(sn_f), 1.0)
+        assert abs(c64 - float(ce)) <= 2 * math.ulp(float(ce)), (u, c64, float(ce))
+        assert abs(s64 - float(se)) <= 2 * math.ulp(float(se)), (u, s64, float(se))

@@ -70,7 +70,7 @@ def test_noalloc_streaming_shape_and_stop():
     for name in ("alloc", "noalloc"):
         with capi.Context(p, precision=capi.XF_FP64, grid=8) as ctx:
             ctx.upload(psi0, 10.0, 1e-3)
-            ctx.set_launch(4, 2, 64 * ((2 * 40 // 2 + 63) // 64), 2)
+            ctx.set_launch(4, 2, capi.block_for(2, 40, 2), 2)
             run = ctx.run if name == "alloc" else ctx.run_noalloc
             info, rec = run(60, log=True, stop=stop)
             out[name] = (info, rec, *ctx.download(), ctx.launch())

@@ -33,7 +33,7 @@ class XfError(RuntimeError):
 class XfOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("species", ctypes.c_int), ("renorm", ctypes.c_int),
                 ("block", ctypes.c_int), ("cpt", ctypes.c_int), ("resident", ctypes.c_int),
-                ("grid", ctypes.c_int)]
+                ("grid", ctypes.c_int), ("mode", ctypes.c_int)]
 
 
 class XfStepRec(ctypes.Structure):
@@ -175,6 +175,13 @@ def dp(a: np.ndarray):
     return a.ctypes.data_as(_DP)
 
 
+def block_for(bpt: int, ne: int, cpt: int) -> int:
+    """Threads of a workgroup whose tile is `bpt` beams of `ne` energy bins at
+    `cpt` cells per lane (the tile's bpt * ne cells over the lanes, rounded up
+    to whole waves): the block xf_set_launch needs for that tile."""
+    return 64 * (((bpt * ne + cpt - 1) // cpt + 63) // 64)
+
+
 def device_count() -> int:
     n = ctypes.c_int()
     load().xf_device_count(ctypes.byref(n))
@@ -301,7 +308,7 @@ class Context:
     def launch(self):
         o = XfOpts(); t = ctypes.c_int(); b = ctypes.c_int()
         check(self.L.xf_get_launch(self.h, ctypes.byref(o), ctypes.byref(t), ctypes.byref(b)))
-        return dict(precision=o.precision, species=o.species, block=o.block, cpt=o.cpt, mode=o.resident,
+        return dict(precision=o.precision, species=o.species, block=o.block, cpt=o.cpt, mode=o.mode,
                     grid=o.grid, renorm=o.renorm, tiles=t.value, beams_per_tile=b.value)
 
     def set_launch(self, mode: int, cpt: int, block: int, bpt: int):

@@ -824,7 +824,8 @@ static int ens_autotune(xf_ensemble* E, int steps, xf_opts* best, double* best_m
         best->species = E->inst[0]->ns;
         best->block = E->L.block;
         best->cpt = E->L.cpt;
-        best->resident = E->L.mode;
+        best->resident = (E->L.mode == 0 || E->L.mode == 1) ? 1 : 0;
+        best->mode = E->L.mode;
         best->grid = E->L.grid;
     }
     if (best_ms_per_step) *best_ms_per_step = best_t;
@@ -1080,7 +1081,8 @@ int xf_get_launch(const xf_ctx* ctx, xf_opts* out, int* tiles, int* beams_per_ti
         out->species = E->inst[0]->ns;
         out->block = E->L.block;
         out->cpt = E->L.cpt;
-        out->resident = E->L.mode;   // 0 keep, 1 recompute, 2 stream
+        out->resident = (E->L.mode == 0 || E->L.mode == 1) ? 1 : 0;
+        out->mode = E->L.mode;      // 0 keep, 1 recompute, 2 / 4 stream
         out->grid = E->L.grid;
         out->renorm = E->inst[0]->renorm;
     }

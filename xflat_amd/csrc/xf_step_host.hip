@@ -63,7 +63,7 @@ bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nu
 // Doubles per beam record in LDS (at least the kernel's BRec size).
 static int record_doubles(int geom) {
     const int nm = geom == XF_MAA ? 4 : geom == XF_SA ? 1 : 2;
-    return 3 + 3 * nm + 9 + 12 + 4;
+    return 3 + 3 * nm + 9 + 12 + 4 + 8;
 }
 
 // The kernel's dynamic LDS, in doubles, in its order: beam records, the
@@ -98,7 +98,8 @@ int xf_step_max_blocks_geom(const XfLaunch& L, int geom, int* max_blocks) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return xf_fail(XF_EHIP, "hipGetDevice failed");
     int ncu = 0;
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        return xf_fail(XF_EHIP, "hipDeviceGetAttribute(multiprocessor count) failed");
     size_t smem = xf_step_smem_bytes(L, geom);
     if (smem > XF_LDS_MAX) return xf_fail(XF_EINVAL, "shape needs %zu B of LDS, the workgroup limit is %zu", smem, XF_LDS_MAX);
     if (smem > 64 * 1024) {

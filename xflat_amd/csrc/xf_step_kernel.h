@@ -34,10 +34,16 @@
 // LDS (for the whole step when they fit, else in chunks of tiles), so the
 // tiles of a chunk need no set-up or barrier of their own.
 //
-// Precision: FP64 mirrors NBeam::U (NBeam.cpp:360-393) operation by
-// operation.  The FP32 modes propagate increments delta = (U - I) psi in
-// FP32, with cos x - 1 and sin(x)/x - 1 as polynomials in x^2 (prop_f2),
-// and estimate the step error from second-order remainders plus an FP64
+// Precision: FP64 forms NBeam::U (NBeam.cpp:360-393) as cos(x) I -
+// i sin(x)/lambda H, x = lambda dl, with cos x and sin(x)/lambda from their
+// Taylor series in x^2 below x^2 = 1/16 (to x^16, and to x^8 / x^9 below
+// x^2 = 2^-10; prop_ds) and from sincos above; where one Hamiltonian acts
+// over dl and 2 dl the longer step comes from the double-angle identities
+// (prop_d2).  The FP32 modes propagate increments delta = (U - I) psi in
+// FP32, with cos x - 1 and sin(x)/x - 1 as polynomials in x^2 (prop_f2s: a
+// wave-uniform linear tier when every lane has x^2 < 2^-10, Chebyshev
+// cubics below x^2 = 1, hardware sin/cos of the half angle above), and
+// estimate the step error from second-order remainders plus an FP64
 // first-order term (pass C); moments, error sums and the decision are FP64.
 #include <hip/hip_runtime.h>
 #include <type_traits>
@@ -265,12 +271,14 @@ template <class T> __device__ __forceinline__ T dnorm2(const Sp<T>& x, const Sp<
 }
 
 // Tagged row words of the cross-workgroup exchange: 32-bit generation tag
-// (exchange index + 1) over one 32-bit half of a partial sum.
+// (exchange index + 1) over one 32-bit half of a partial sum, accessed
+// through the global address space (global, not flat, loads and stores).
+using gword = __attribute__((address_space(1))) unsigned long long;
 __device__ __forceinline__ unsigned long long ld_word(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load((const gword*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_word(unsigned long long* p, unsigned long long w) {
-    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((gword*)p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Bounded-wait bookkeeping: 2 s of the 100 MHz real-time counter, or an
 // error flagged by another workgroup.  Returns 0 while the wait may go on,
@@ -344,7 +352,11 @@ template <int GEOM> struct BRec {
     static constexpr int G = W + 3 * NM;          // G[3 pts][3] hvv geometry
     static constexpr int H = G + 9;               // Hb[4][3]: h0, h1, h3, h4 (+matter)
     static constexpr int HD = H + 12;             // dl0 Hb0 + dl2 Hb4 + dl1 (Hb3 - Hb0 - Hb1), dl0 + dl2 - dl1
-    static constexpr int SIZE = HD + 4;
+    // FP32 copies for the FP32-increment steppers: Hb[4][3] in floats 0..11,
+    // dl[3] in floats 12..14 (a cell then forms its Hamiltonian with two FP32
+    // adds instead of two FP64 adds and three conversions per propagator)
+    static constexpr int HF = HD + 4;
+    static constexpr int SIZE = HF + 8;
 };
 
 // Geometry of one beam at one of the step's points r, r+dr/2, r+dr (pt 0,
@@ -415,15 +427,21 @@ __device__ __forceinline__ void beam_dl(double* rec, double dr, double dr2) {
         rec[R::DL + 1] = dr / c0;
         rec[R::DL + 2] = dr2 / c1;
     }
+    float* f = reinterpret_cast<float*>(rec + R::HF) + 12;
+    f[0] = (float)rec[R::DL + 0];
+    f[1] = (float)rec[R::DL + 1];
+    f[2] = (float)rec[R::DL + 2];
 }
 
 // H_vv of one beam at point pt from the moments M[m*3+z] (z: density comps
 // 0,2,3): Phy_MA.cpp:372-377 / Phy_MAA.cpp:445-461; plus the matter term on
-// component 0 (NBeam.cpp:604).
+// component 0 (NBeam.cpp:604); into Hamiltonian slot `slot` of the record
+// (0..3: h0, h1, h3, h4) and its FP32 copy.
 template <int GEOM>
-__device__ __forceinline__ void beam_h(const double* rec, int pt, const double* M, double coef, double hm, double* Hb) {
+__device__ __forceinline__ void beam_h(double* rec, int pt, int slot, const double* M, double coef, double hm) {
     using R = BRec<GEOM>;
     const double* G = rec + R::G + pt * 3;
+    double* Hb = rec + R::H + slot * 3;
     double hv[3];
 #pragma unroll
     for (int z = 0; z < 3; ++z) {
@@ -437,6 +455,10 @@ __device__ __forceinline__ void beam_h(const double* rec, int pt, const double* 
     Hb[0] = hv[0] + hm;
     Hb[1] = hv[1];
     Hb[2] = hv[2];
+    float* f = reinterpret_cast<float*>(rec + R::HF) + slot * 3;
+    f[0] = (float)Hb[0];
+    f[1] = (float)Hb[1];
+    f[2] = (float)Hb[2];
 }
 
 // First-order combination of the step's four Hamiltonians that enters
@@ -483,11 +505,13 @@ __device__ __forceinline__ void st_sp(ST* __restrict__ base, long long ncells, i
 
 // Per-energy table in LDS, one record of tab_slots(NS) 8-byte slots per
 // energy bin: H_vac (h_r0, h_r1), the error weights fv of the 4 particles,
-// and the NS source weights (FP32 ones in the low half of their slot).  A
-// cell reads every field at an immediate offset from one 32-bit address.
-// The record length is odd in 8-byte units, so the lanes of a wave, which
-// hold consecutive energy bins, read 32 distinct 8-byte banks in each half
-// wave (an even length would fold them onto a few banks).
+// the NS source weights (FP32 ones in the low half of their slot), and for
+// the FP32-increment steppers H_vac as two floats in slot 6 + NS (the
+// padding slot: NS is 2 or 4).  A cell reads every field at an immediate
+// offset from one 32-bit address.  The record length is odd in 8-byte units,
+// so the lanes of a wave, which hold consecutive energy bins, read 32
+// distinct 8-byte banks in each half wave (an even length would fold them
+// onto a few banks).
 constexpr int tab_slots(int ns) { return (6 + ns) | 1; }
 
 // Hamiltonian of species s at energy e for beam H record Hb (NBeam.cpp:602-625):
@@ -504,6 +528,22 @@ __device__ __forceinline__ void cell_h(const double* te, int s, const double* Hb
         h1 = hv1 + Hb[1];
     }
     h2 = Hb[2];
+}
+
+// The same in FP32 for the FP32-increment steppers, from the FP32 copies of
+// H_vac (tef, in the energy's table record) and of Hb (Hf, in the beam's
+// record).  The propagators take FP32 Hamiltonians in any case; the error
+// estimate's first-order term keeps the FP64 ones (beam_hd).
+__device__ __forceinline__ void cell_hf(const float* tef, int s, const float* Hf, float& h0, float& h1, float& h2) {
+    const float hv0 = tef[0], hv1 = tef[1];
+    if (s & 1) {
+        h0 = hv0 + (-Hf[0]);
+        h1 = hv1 + (-Hf[1]);
+    } else {
+        h0 = hv0 + Hf[0];
+        h1 = hv1 + Hf[1];
+    }
+    h2 = Hf[2];
 }
 
 // Source contribution of one cell to the angular integral, X(z) for density
@@ -839,6 +879,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     constexpr int NM = R::NM;
     constexpr int NMV = NM * 3;
     static_assert(2 * NMV + 1 <= XF_NVMAX, "partials");
+    static_assert(NS == 2 || NS == 4, "the table's padding slot holds the FP32 H_vac");
 
     const XfInstDev& I = insts[blk_inst[blockIdx.x]];
     const int B = blockDim.x;
@@ -887,6 +928,11 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         t[1] = I.hv[ne + e];
 #pragma unroll
         for (int q = 0; q < 4; ++q) t[2 + q] = I.errw[q * ne + e];
+        if constexpr (!F64) {
+            float* tf = reinterpret_cast<float*>(t + 6 + NS);
+            tf[0] = (float)I.hv[e];
+            tf[1] = (float)I.hv[ne + e];
+        }
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
             if constexpr (F64) {
@@ -995,7 +1041,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         const int nq = min(tch, ntl - c0) * bpt;
         for (int q = tid; q < nq; q += B)
             if (table_beam(c0, q) < I.nbeam)
-                beam_h<GEOM>(s_beam + q * R::SIZE, pt, M, coef, hm, s_beam + q * R::SIZE + R::H + slot * 3);
+                beam_h<GEOM>(s_beam + q * R::SIZE, pt, slot, M, coef, hm);
     };
     auto table_hd = [&](int c0) __attribute__((always_inline)) {
         const int nq = min(tch, ntl - c0) * bpt;
@@ -1004,7 +1050,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     };
     auto tile_h = [&](int nbt, int slot, int pt, const double* M, double hm) __attribute__((always_inline)) {
         const double coef = F64 ? I.coef : I.coef_f;   // float source weights are pre-scaled
-        for (int b = tid; b < nbt; b += B) beam_h<GEOM>(s_beam + b * R::SIZE, pt, M, coef, hm, s_beam + b * R::SIZE + R::H + slot * 3);
+        for (int b = tid; b < nbt; b += B) beam_h<GEOM>(s_beam + b * R::SIZE, pt, slot, M, coef, hm);
     };
 
     const int my_tile0 = lb;                 // resident: exactly this tile
@@ -1092,6 +1138,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     const double* te = s_tab + e * TS;
+                    const float* tef = reinterpret_cast<const float*>(te + 6 + NS);
+                    const float* recf = reinterpret_cast<const float*>(rec + R::HF);
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0), Y0 = CT(0), Y2 = CT(0), Y3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
@@ -1113,9 +1161,10 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             src_add<CT>(te, s, v3, Y0, Y2, Y3);
                         } else if constexpr (DBL) {
                             // dl1 = 2 dl0: U(dl1) - I = 2 D0 + D0^2
-                            const float f0 = (float)h0, f1 = (float)h1, f2 = (float)h2;
+                            float f0, f1, f2;
+                            cell_hf(tef, s, recf + 0, f0, f1, f2);
                             float cm1, sc, scr;
-                            prop_f2s(f0, f1, f2, (float)rec[R::DL + 0], cm1, sc, scr);
+                            prop_f2s(f0, f1, f2, recf[12], cm1, sc, scr);
                             const PropF P0 = pf_of(f0, f1, f2, cm1, sc);
                             Sp<float> a = sp_f(C.p0[k][s]);
                             Sp<float> d3 = delta_f(P0, a);
@@ -1131,8 +1180,10 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             src_add<CT>(te, s, sp_add(a, d3), Y0, Y2, Y3);
                         } else {
                             PropF P1, P0, PR1, PR0;
-                            prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], P1, PR1);
-                            prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 0], P0, PR0);
+                            float f0, f1, f2;
+                            cell_hf(tef, s, recf + 0, f0, f1, f2);
+                            prop_f2(f0, f1, f2, recf[13], P1, PR1);
+                            prop_f2(f0, f1, f2, recf[12], P0, PR0);
                             Sp<float> a = sp_f(C.p0[k][s]);
                             Sp<float> d1 = delta_f(P1, a), d3 = delta_f(P0, a);
                             if (MODE == 0) {
@@ -1198,6 +1249,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     const double* te = s_tab + e * TS;
+                    const float* tef = reinterpret_cast<const float*>(te + 6 + NS);
+                    const float* recf = reinterpret_cast<const float*>(rec + R::HF);
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
@@ -1217,11 +1270,14 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
                             PropF Pb, P4, PRb, PR4;
-                            prop_f2((float)g0, (float)g1, (float)g2, (float)rec[R::DL + 1], P4, PR4);
+                            float f0, f1, f2;
+                            cell_hf(tef, s, recf + 6, f0, f1, f2);
+                            prop_f2(f0, f1, f2, recf[13], P4, PR4);
                             Sp<float> d4 = delta_f(P4, a);
                             if (MODE == 0) {
                                 // Q -= Rb, Q += R3 (the order of the recompute path)
-                                prop_f2((float)h0, (float)h1, (float)h2, (float)rec[R::DL + 1], Pb, PRb);
+                                cell_hf(tef, s, recf + 3, f0, f1, f2);
+                                prop_f2(f0, f1, f2, recf[13], Pb, PRb);
                                 const float4 o = s_keep[(k * NS + s) * B + tid];
                                 const PropF q = pf_add(pf_sub(PropF{o.x, o.y, o.z, o.w}, PRb), PR4);
                                 s_keep[(k * NS + s) * B + tid] = make_float4(q.cm1, q.m1, q.m2, q.m3);
@@ -1295,6 +1351,8 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     const double* rec = tb + C.bl[k] * R::SIZE;
                     const int e = C.e[k];
                     const double* te = s_tab + e * TS;
+                    const float* tef = reinterpret_cast<const float*>(te + 6 + NS);
+                    const float* recf = reinterpret_cast<const float*>(rec + R::HF);
                     CT X0 = CT(0), X2 = CT(0), X3 = CT(0);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
@@ -1358,31 +1416,33 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                     d3 = C.k3[k][s];
                                     d4 = C.k4[k][s];
                                 } else {
-                                double x0, x1, x2;
-                                cell_h(te, s, rec + R::H + 0, x0, x1, x2);                    // h0
+                                float f0, f1, f2;
+                                cell_hf(tef, s, recf + 0, f0, f1, f2);                         // h0
                                 if constexpr (DBL) {
-                                    const float f0 = (float)x0, f1 = (float)x1, f2 = (float)x2;
                                     float cm1, sc, scr;
-                                    prop_f2s(f0, f1, f2, (float)rec[R::DL + 0], cm1, sc, scr);
+                                    prop_f2s(f0, f1, f2, recf[12], cm1, sc, scr);
                                     d3 = delta_f(pf_of(f0, f1, f2, cm1, sc), a);
                                     Q = pf_rdiff2(f0, f1, f2, cm1, sc, scr);
                                 } else {
-                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
+                                prop_f2(f0, f1, f2, recf[13], P, PR);
                                 Q = PropF{0.f - PR.cm1, 0.f - PR.m1, 0.f - PR.m2, 0.f - PR.m3};
-                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 0], P, PR);
+                                prop_f2(f0, f1, f2, recf[12], P, PR);
                                 d3 = delta_f(P, a);
                                 Q = pf_add(Q, PR);
                                 }
-                                cell_h(te, s, rec + R::H + 3, x0, x1, x2);                    // h1
-                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
+                                cell_hf(tef, s, recf + 3, f0, f1, f2);                         // h1
+                                prop_f2(f0, f1, f2, recf[13], P, PR);
                                 Q = pf_sub(Q, PR);
-                                cell_h(te, s, rec + R::H + 6, x0, x1, x2);                    // h3
-                                prop_f2((float)x0, (float)x1, (float)x2, (float)rec[R::DL + 1], P, PR);
+                                cell_hf(tef, s, recf + 6, f0, f1, f2);                         // h3
+                                prop_f2(f0, f1, f2, recf[13], P, PR);
                                 d4 = delta_f(P, a);
                                 Q = pf_add(Q, PR);
                                 }
-                                const float dl2 = (float)rec[R::DL + 2];
-                                prop_f2((float)h0, (float)h1, (float)h2, dl2, P, PR);           // h4
+                                {
+                                    float f0, f1, f2;
+                                    cell_hf(tef, s, recf + 9, f0, f1, f2);
+                                    prop_f2(f0, f1, f2, recf[14], P, PR);                      // h4
+                                }
                                 const Sp<float> u = delta_f(P, d3);
                                 const Sp<float> d3n = sp_add(d3, sp_add(delta_f(P, a), u));
                                 const double* hd = rec + R::HD;

@@ -146,7 +146,7 @@ int main(int argc, char** argv) {
         double ms = 0.;
         CHECK(xf_autotune(ctx, autotune, &best, &ms));
         if (!quiet)
-            std::printf("Launch shape: mode %d, %d cells per thread, %d threads (%.4f ms per step)\n", best.resident,
+            std::printf("Launch shape: mode %d, %d cells per thread, %d threads (%.4f ms per step)\n", best.mode,
                         best.cpt, best.block, ms);
     }
 
