@@ -73,11 +73,12 @@ struct XfRunScalars {
 };
 
 // Cross-workgroup reduction of one instance (all-gather of tagged rows):
-// each workgroup publishes its partial sums as one row of 64-bit words, value
-// v split into its low and high 32-bit halves (words 2v, 2v + 1), each word
-// carrying the exchange's generation + 1 in its upper 32 bits.  Rows are
-// value-major, rows[parity][word][nblk], so word w of all rows is one
-// contiguous array.  Every workgroup fetches every row and reduces them in
+// each workgroup publishes its partial sums as one row of 64-bit words, each
+// word carrying the exchange's generation + 1: the FP64 stepper splits value
+// v into its low and high 32-bit halves (words 2v, 2v + 1) under a 32-bit
+// tag, the FP32-increment steppers put value v in word v with the tag's three
+// low bits in its three low mantissa bits.  Rows are value-major,
+// rows[parity][word][nblk], so word w of all rows is one contiguous array.  Every workgroup fetches every row and reduces them in
 // the same order.  Generations continue from launch to launch (the run
 // scalars' gen0): a slot then holds the tag of its current exchange or of the
 // one two before it, never a later one.  The rows are cleared and the count
@@ -85,7 +86,7 @@ struct XfRunScalars {
 // new state, a failed launch, the no-allocation path's own rows) or where the
 // tags would pass XF_MAX_EXCHANGES.
 constexpr int XF_NVMAX = 32;                   // max values per partial row
-constexpr int XF_ROWW = 2 * XF_NVMAX;          // words per row
+constexpr int XF_ROWW = 2 * XF_NVMAX;          // words per row (at most)
 constexpr int XF_ROW_REPLICAS = 4;             // copies of every row (reads spread over more lines)
 constexpr long long XF_MAX_EXCHANGES = 0xF0000000ll;
 

@@ -161,8 +161,10 @@ int xf_step_launch_geom(const XfLaunch& L, int geom, const XfInstDev* d_insts, c
 }
 
 // Status and exchange rows of every instance to zero and the generation
-// count restarted (generation tags start at 1, so a cleared word matches no
-// exchange).
+// count restarted.  A cleared word matches no 32-bit tag (they start at 1);
+// its three low bits match the one-word format's tags of generations 7, 15,
+// ..., but a slot is first read at the exchange that first writes it, with
+// generation 0 to 3, so a cleared word is never taken.
 __global__ void k_step_prepare(const XfInstDev* __restrict__ insts) {
     const XfInstDev& I = insts[blockIdx.y];
     if (blockIdx.x == 0 && threadIdx.x == 0) {

@@ -270,15 +270,48 @@ template <class T> __device__ __forceinline__ T dnorm2(const Sp<T>& x, const Sp<
     return (dar * dar + dai * dai) + (dbr * dbr + dbi * dbi);
 }
 
-// Tagged row words of the cross-workgroup exchange: 32-bit generation tag
-// (exchange index + 1) over one 32-bit half of a partial sum, accessed
-// through the global address space (global, not flat, loads and stores).
+// Tagged row words of the cross-workgroup exchange, 64-bit single-copy-atomic
+// stores and loads through the global address space (global, not flat,
+// instructions).  The tag is the exchange's generation + 1; a slot holds the
+// word of its current exchange or of the one two before it (rows[gen & 1]).
+// Two formats:
+//  - two words per value (W1 = false, the FP64 stepper, whose moments enter
+//    the FP64 parity with the reference unrounded): the value's low and high
+//    32-bit halves, each under the 32-bit tag;
+//  - one word per value (W1 = true, the FP32-increment steppers): the value
+//    rounded to the nearest multiple of 8 units in its last place (relative
+//    change below 2^-50, far below their FP32 state), its three low mantissa
+//    bits replaced by the tag's low three bits.  Half the words to store and
+//    fetch.  A step's exchanges carry 2 NMV, NMV and NMV + 1 values, so a
+//    word that only some of them write was last written 2, 4 or 6
+//    generations before (never a multiple of 8), and a cleared word (bits 0)
+//    is never read before the first write of its slot.
 using gword = __attribute__((address_space(1))) unsigned long long;
 __device__ __forceinline__ unsigned long long ld_word(const unsigned long long* p) {
     return __hip_atomic_load((const gword*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_word(unsigned long long* p, unsigned long long w) {
     __hip_atomic_store((gword*)p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool W1> constexpr int words_per_value() { return W1 ? 1 : 2; }
+// word q of value x
+template <bool W1> __device__ __forceinline__ unsigned long long word_of(double x, unsigned tag, int q) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    if constexpr (W1) return ((u + 4ull) & ~7ull) | (unsigned long long)(tag & 7u);
+    else return ((unsigned long long)tag << 32) | (q ? (u >> 32) : (u & 0xffffffffull));
+}
+// the tag bits of a word (for a 32-bit load, the half that holds them)
+template <bool W1> __device__ __forceinline__ bool tag_current(unsigned long long w, unsigned tag) {
+    if constexpr (W1) return ((unsigned)w & 7u) == (tag & 7u);
+    else return (unsigned)(w >> 32) == tag;
+}
+template <bool W1> __device__ __forceinline__ unsigned tag_bits(unsigned long long w) {
+    return W1 ? ((unsigned)w & 7u) : (unsigned)(w >> 32);
+}
+// the value from its words w0 (and w1)
+template <bool W1> __device__ __forceinline__ double word_value(unsigned long long w0, unsigned long long w1) {
+    if constexpr (W1) return __longlong_as_double((long long)(w0 & ~7ull));
+    else return __longlong_as_double((long long)((w1 << 32) | (w0 & 0xffffffffull)));
 }
 // Bounded-wait bookkeeping: 2 s of the 100 MHz real-time counter, or an
 // error flagged by another workgroup.  Returns 0 while the wait may go on,
@@ -296,18 +329,19 @@ __device__ __forceinline__ int wait_expired(uint64_t t0, int spins, int* status)
 // After this wave's own wait expired: what it saw in the word array w of the
 // n rows it waited on (the rows whose tag is not `tag`, and the tag the first
 // of them holds), kept by the first workgroup to get here.  One full wave.
+template <bool W1>
 __device__ __noinline__ void note_expiry(XfRunScalars* rs, int where, int lb, unsigned tag,
                                          const unsigned long long* w, int n) {
     const int lane = threadIdx.x & 63;
     int missing = 0, first = n;
     for (int j0 = 0; j0 < n; j0 += 64) {
         const int j = j0 + lane;
-        const bool bad = j < n && (unsigned)(ld_word(w + min(j, n - 1)) >> 32) != tag;
+        const bool bad = j < n && !tag_current<W1>(ld_word(w + min(j, n - 1)), tag);
         const unsigned long long m = __ballot(bad);
         missing += __popcll(m);
         if (m && first == n) first = j0 + __ffsll((long long)m) - 1;
     }
-    const unsigned seen = first < n ? (unsigned)(ld_word(w + first) >> 32) : tag;
+    const unsigned seen = first < n ? tag_bits<W1>(ld_word(w + first)) : tag;
     if (lane == 0) {
         int expect = 0;
         if (__hip_atomic_compare_exchange_strong(&rs->diag_where, &expect, where, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -664,16 +698,16 @@ __device__ __forceinline__ long long uni_ll(long long v) {
 //     values when NT * B exceeds it); wave w reduces values w, w + nwaves,
 //     ...: lane l sums lanes l, l + 64, ... in order, then the DPP wave
 //     reduction, and the wave publishes the total as row lb: value v ->
-//     words 2v (low half) and 2v + 1 (high half), each tagged with gen + 1,
-//     into every replica of rows[gen & 1][replica][word][nblk];
+//     its words (word v, or words 2v and 2v + 1), tagged with gen + 1
+//     (word_of), into every replica of rows[gen & 1][replica][word][nblk];
 //  2. wave 0 polls word 0 of all rows of replica lb % XF_ROW_REPLICAS (one
 //     contiguous array, each lane's loads in flight together) until every
 //     row carries the tag;
-//  3. each wave reads both halves of its values (two at a time) for all rows
-//     of that replica straight into registers (16 loads per lane in flight),
-//     checks every tag (re-reading on a stale one) and sums each value over
-//     the rows in a fixed order (lane l summing rows l, l + 64, ..., then
-//     the DPP reduction), which leaves the same bits in every workgroup.
+//  3. each wave reads its values (two at a time) for all rows of that
+//     replica straight into registers (8 or 16 loads per lane in flight), checks
+//     every tag (re-reading on a stale one) and sums each value over the
+//     rows in a fixed order (lane l summing rows l, l + 64, ..., then the
+//     DPP reduction), which leaves the same bits in every workgroup.
 // Replicas spread the all-gather's reads, which every workgroup makes of the
 // same few kilobytes, over more memory lines.  The rows are the only data
 // that crosses workgroups, so no fence is needed: each word is one
@@ -684,7 +718,7 @@ __device__ __forceinline__ long long uni_ll(long long v) {
 // ones).  All waits are bounded.
 // Step 1 (xch_publish) and steps 2-4 (xch_gather), run back to back by
 // reduce_publish.
-template <int NV, int NMAX>
+template <int NV, int NMAX, bool W1>
 __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int xcap, const XfInstDev& I, int lb,
                                             unsigned gen, long long* tr = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
@@ -693,7 +727,6 @@ __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int 
     constexpr int NT = NV + NMAX;
     constexpr int RP = XF_ROW_REPLICAS;
     const unsigned tag = gen + 1u;
-    const unsigned long long tagw = (unsigned long long)tag << 32;
     const size_t rep_words = (size_t)XF_ROWW * nblk;
     unsigned long long* rows0 = I.rows + (size_t)(gen & 1u) * RP * rep_words;
     if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
@@ -723,11 +756,11 @@ __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int 
                 const int v = v0 + cc[h];
                 if (cc[h] >= nv) break;               // wave-uniform
                 x[h] = lane63(v < NV ? wave_red_dpp<false>(x[h]) : wave_red_dpp<true>(x[h]));
-                if (lane >= 64 - 2 * RP) {
-                    const int q = 63 - lane;
-                    const unsigned long long u = (unsigned long long)__double_as_longlong(x[h]);
-                    const unsigned half = (q & 1) ? (unsigned)(u >> 32) : (unsigned)u;
-                    st_word(rows0 + (size_t)(q >> 1) * rep_words + (size_t)(2 * v + (q & 1)) * nblk + lb, tagw | half);
+                constexpr int WV = words_per_value<W1>();
+                if (lane >= 64 - WV * RP) {
+                    const int q = 63 - lane;                  // replica q / WV, word q % WV
+                    st_word(rows0 + (size_t)(q / WV) * rep_words + (size_t)(WV * v + q % WV) * nblk + lb,
+                            word_of<W1>(x[h], tag, q % WV));
                 }
             }
         }
@@ -738,21 +771,23 @@ __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int 
 // Whether word 0 of rows 0 .. n-1 (n <= 256) carries the tag: lane l reads
 // rows l + 64 u for u < 4, four loads in flight and none waiting for
 // another; indexes past the last row read the last row again.
-// Only the tag half (the upper 32 bits, at byte offset 4) is read, through
-// a global address-space pointer.
+// Only the 32-bit half that holds the tag bits is read (the low one for the
+// one-word format, the high one for the two-word format), through a global
+// address-space pointer.
+template <bool W1>
 __device__ __forceinline__ int rows_current(const unsigned long long* rows, int lane, int n, unsigned tag) {
     using gtag = const __attribute__((address_space(1))) unsigned int;
-    gtag* t = (gtag*)(reinterpret_cast<const unsigned int*>(rows) + 1);
+    gtag* t = (gtag*)(reinterpret_cast<const unsigned int*>(rows) + (W1 ? 0 : 1));
     unsigned w[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) w[u] = __hip_atomic_load(t + 2 * min(lane + 64 * u, n - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int mine = 1;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) mine &= w[u] == tag;
+    for (int u = 0; u < 4; ++u) mine &= tag_current<W1>(W1 ? (unsigned long long)w[u] : (unsigned long long)w[u] << 32, tag);
     return mine;
 }
 
-template <int NV, int NMAX>
+template <int NV, int NMAX, bool W1>
 __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, int lb, unsigned& gen,
                                            int* status, int* s_flag, long long* tr = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
@@ -761,7 +796,8 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
     constexpr int NT = NV + NMAX;
     constexpr int RP = XF_ROW_REPLICAS;
     const unsigned tag = gen + 1u;
-    const unsigned long long tagw = (unsigned long long)tag << 32;
+    const unsigned long long tagw = word_of<W1>(0., tag, 0);             // a current word (for absent rows)
+    constexpr int WV = words_per_value<W1>();
     const size_t rep_words = (size_t)XF_ROWW * nblk;
     unsigned long long* rows = I.rows + (size_t)(gen & 1u) * RP * rep_words + (size_t)(lb % RP) * rep_words;
     if (tid < 64) {
@@ -769,10 +805,10 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
         int fail = 0;
         for (int spins = 1;; ++spins) {
             int mine = 1;
-            for (int j0 = 0; j0 < nblk; j0 += 256) mine &= rows_current(rows + j0, lane, min(nblk - j0, 256), tag);
+            for (int j0 = 0; j0 < nblk; j0 += 256) mine &= rows_current<W1>(rows + j0, lane, min(nblk - j0, 256), tag);
             if (__all(mine)) break;
             if (const int x = wait_expired(t0, spins, status)) {
-                if (x == 1) note_expiry(I.rs, 1, lb, tag, rows, nblk);
+                if (x == 1) note_expiry<W1>(I.rs, 1, lb, tag, rows, nblk);
                 fail = 1;
                 break;
             }
@@ -783,11 +819,10 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
     __syncthreads();
     int fail = s_flag[0];
     // fetch and totals: wave w takes values w, w + nwaves, ... two at a
-    // time; lane l reads both halves of rows l, l + 64, l + 128, l + 192 of
-    // each (16 loads in flight per chunk of 256 rows) and adds them in row
-    // order, which is the fixed order of the sum over rows.  A stale tag (a
-    // word that had not landed when word 0 had) makes the wave read the chunk
-    // again.
+    // time; lane l reads rows l, l + 64, l + 128, l + 192 of each (8 or 16
+    // loads in flight per chunk of 256 rows) and adds them in row order, which is the
+    // fixed order of the sum over rows.  A stale tag (a word that had not
+    // landed when word 0 had) makes the wave read the chunk again.
     const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
     int wfail = fail;
     for (int v0 = wave; v0 < NT; v0 += 2 * nwaves) {
@@ -796,24 +831,24 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
         double x[2] = {0., 0.};
         for (int j0 = 0; j0 < nblk && !wfail; j0 += 256) {
             for (int spins = 1;; ++spins) {
-                unsigned long long w[2][2][4];                // [value][half][row]
+                unsigned long long w[2][WV][4];               // [value][word][row]
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
-                    for (int q = 0; q < 2; ++q)
+                    for (int q = 0; q < WV; ++q)
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             const int j = j0 + lane + 64 * u;
                             const int v = h ? v1 : v0;
-                            w[h][q][u] = ((h == 0 || two) && j < nblk) ? ld_word(rows + (size_t)(2 * v + q) * nblk + j) : tagw;
+                            w[h][q][u] = ((h == 0 || two) && j < nblk) ? ld_word(rows + (size_t)(WV * v + q) * nblk + j) : tagw;
                         }
                 int bad = 0;
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
-                    for (int q = 0; q < 2; ++q)
+                    for (int q = 0; q < WV; ++q)
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) bad |= (unsigned)(w[h][q][u] >> 32) != tag;
+                        for (int u = 0; u < 4; ++u) bad |= !tag_current<W1>(w[h][q][u], tag);
                 if (!__any(bad)) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
@@ -821,15 +856,14 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             if (j0 + lane + 64 * u >= nblk) continue;
-                            const double y = __longlong_as_double(
-                                (long long)((w[h][1][u] << 32) | (w[h][0][u] & 0xffffffffull)));
+                            const double y = word_value<W1>(w[h][0][u], w[h][WV - 1][u]);
                             x[h] = is_sum ? x[h] + y : fmax(x[h], y);
                         }
                     }
                     break;
                 }
                 if (const int x = wait_expired(t1, spins, status)) {
-                    if (x == 1) note_expiry(I.rs, 2, lb, tag, rows + (size_t)(2 * v0) * nblk, nblk);
+                    if (x == 1) note_expiry<W1>(I.rs, 2, lb, tag, rows + (size_t)(WV * v0) * nblk, nblk);
                     wfail = 1;
                     break;
                 }
@@ -849,11 +883,11 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
     return !fail;
 }
 
-template <int NV, int NMAX>
+template <int NV, int NMAX, bool W1>
 __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot, double* s_x, int xcap, const XfInstDev& I,
                                                int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr) {
-    xch_publish<NV, NMAX>(acc, s_x, xcap, I, lb, gen, tr);
-    return xch_gather<NV, NMAX>(s_tot, I, lb, gen, status, s_flag, tr);
+    xch_publish<NV, NMAX, W1>(acc, s_x, xcap, I, lb, gen, tr);
+    return xch_gather<NV, NMAX, W1>(s_tot, I, lb, gen, status, s_flag, tr);
 }
 }  // namespace
 
@@ -1202,7 +1236,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
                 }
             }
-            ok = reduce_publish<2 * NMV, 0>(acc, s_MA, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
+            ok = reduce_publish<2 * NMV, 0, !F64>(acc, s_MA, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
             if (!ok) break;
         }
 
@@ -1289,7 +1323,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
                 }
             }
-            ok = reduce_publish<NMV, 0>(acc, s_MB, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
+            ok = reduce_publish<NMV, 0, !F64>(acc, s_MB, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
             if (!ok) break;
         }
 
@@ -1509,7 +1543,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (tr && tid == 0) tr[15] = (long long)__builtin_amdgcn_s_memrealtime();
             }
             acc[NMV] = blk_err;
-            ok = reduce_publish<NMV, 1>(acc, Mn, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
+            ok = reduce_publish<NMV, 1, !F64>(acc, Mn, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
             errmax = uni_d(Mn[NMV]);
             if (!ok) break;
         }
