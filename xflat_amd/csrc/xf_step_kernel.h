@@ -787,9 +787,11 @@ __device__ __forceinline__ int rows_current(const unsigned long long* rows, int 
     return mine;
 }
 
-template <int NV, int NMAX, bool W1>
+// `idle`: work that needs none of the totals, run by the other waves while
+// wave 0 polls (by wave 0 after its poll when the block is one wave).
+template <int NV, int NMAX, bool W1, class Idle>
 __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, int lb, unsigned& gen,
-                                           int* status, int* s_flag, long long* tr = nullptr) {
+                                           int* status, int* s_flag, long long* tr, Idle&& idle) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int B = blockDim.x;
     const int nblk = I.nblk;
@@ -814,7 +816,13 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
             }
         }
         fail = __any(fail);
-        if (tid == 0) s_flag[0] = fail;
+        if (tid == 0) {
+            s_flag[0] = fail;
+            s_flag[1] = 0;                    // set below by any wave whose fetch fails
+        }
+        if (nwaves == 1) idle();
+    } else {
+        idle();
     }
     __syncthreads();
     int fail = s_flag[0];
@@ -877,17 +885,21 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
         }
     }
     if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
-    fail = __syncthreads_or(wfail);
+    if (wfail) s_flag[1] = 1;
+    __syncthreads();
+    fail = s_flag[1];
     gen += 1u;
     if (tr && tid == 0) tr[3] = (long long)__builtin_amdgcn_s_memrealtime();
     return !fail;
 }
 
-template <int NV, int NMAX, bool W1>
+struct NoIdle { __device__ void operator()() const {} };
+template <int NV, int NMAX, bool W1, class Idle = NoIdle>
 __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot, double* s_x, int xcap, const XfInstDev& I,
-                                               int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr) {
+                                               int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr,
+                                               Idle&& idle = Idle{}) {
     xch_publish<NV, NMAX, W1>(acc, s_x, xcap, I, lb, gen, tr);
-    return xch_gather<NV, NMAX, W1>(s_tot, I, lb, gen, status, s_flag, tr);
+    return xch_gather<NV, NMAX, W1>(s_tot, I, lb, gen, status, s_flag, tr, idle);
 }
 }  // namespace
 
@@ -979,6 +991,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     }
     __syncthreads();
     __shared__ int s_flag[2];
+    __shared__ double s_hm[2];                                // matter term at r + dr/2, r + dr
     // angular moments, uniform over the workgroup, gathered in place by the
     // exchanges: psi1 and psi3 (pass A), psi4 (pass B), and two buffers that
     // take turns as psi0 and the candidate (pass C, with the step error
@@ -1131,10 +1144,13 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         if (tr && tid == 0) tr[0] = (long long)__builtin_amdgcn_s_memrealtime();
         // matter term: at r carried from the previous step (after an accepted
         // step r is the same double as its r + dr); at r + dr/2 and r + dr
-        // evaluated on lanes 0 and 1 together and broadcast
+        // (needed from pass B on) evaluated on lanes 0 and 1 of wave 1 while
+        // wave 0 polls in pass A's exchange
         const double hm_r = hm_cur;
-        const double hm_l = hm_at(I, (lane & 1) ? rdr : rdr2);
-        const double hm_rdr2 = lane_bcast<0>(hm_l), hm_rdr = lane_bcast<1>(hm_l);
+        double hm_rdr2 = 0., hm_rdr = 0.;
+        auto hm_next = [&]() __attribute__((always_inline)) {
+            if (wave == min(1, nwaves - 1) && lane < 2) s_hm[lane] = hm_at(I, lane ? rdr : rdr2);
+        };
 
         // =================================================== pass A (S1, S2, S6)
         {
@@ -1236,8 +1252,11 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
                 }
             }
-            ok = reduce_publish<2 * NMV, 0, !F64>(acc, s_MA, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr);
+            ok = reduce_publish<2 * NMV, 0, !F64>(acc, s_MA, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr,
+                                                  hm_next);
             if (!ok) break;
+            hm_rdr2 = uni_d(s_hm[0]);
+            hm_rdr = uni_d(s_hm[1]);
         }
 
         // =================================================== pass B (S3, S4/S5, S7, S8)
