@@ -63,7 +63,7 @@ bool xf_step_has_shape(const XfLaunch& L, int geom) { return pick(L, geom) != nu
 // Doubles per beam record in LDS (at least the kernel's BRec size).
 static int record_doubles(int geom) {
     const int nm = geom == XF_MAA ? 4 : geom == XF_SA ? 1 : 2;
-    return 3 + 3 * nm + 9 + 12 + 4 + 8;
+    return 3 + 3 * nm + 9 + 12 + 4 + 10;
 }
 
 // The kernel's dynamic LDS, in doubles, in its order: beam records, the

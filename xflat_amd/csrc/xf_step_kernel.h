@@ -387,10 +387,11 @@ template <int GEOM> struct BRec {
     static constexpr int H = G + 9;               // Hb[4][3]: h0, h1, h3, h4 (+matter)
     static constexpr int HD = H + 12;             // dl0 Hb0 + dl2 Hb4 + dl1 (Hb3 - Hb0 - Hb1), dl0 + dl2 - dl1
     // FP32 copies for the FP32-increment steppers: Hb[4][3] in floats 0..11,
-    // dl[3] in floats 12..14 (a cell then forms its Hamiltonian with two FP32
-    // adds instead of two FP64 adds and three conversions per propagator)
+    // dl[3] in floats 12..14, Hd[4] in floats 16..19 (a cell then forms its
+    // Hamiltonians with two FP32 adds instead of two FP64 adds and three
+    // conversions per propagator)
     static constexpr int HF = HD + 4;
-    static constexpr int SIZE = HF + 8;
+    static constexpr int SIZE = HF + 10;
 };
 
 // Geometry of one beam at one of the step's points r, r+dr/2, r+dr (pt 0,
@@ -510,6 +511,9 @@ __device__ __forceinline__ void beam_hd(double* rec) {
 #pragma unroll
     for (int z = 0; z < 3; ++z) rec[R::HD + z] = dl0 * B0[z] + dl2 * B4[z] + dl1 * ((B3[z] - B0[z]) - B1[z]);
     rec[R::HD + 3] = (dl0 + dl2) - dl1;
+    float* f = reinterpret_cast<float*>(rec + R::HF) + 16;
+#pragma unroll
+    for (int z = 0; z < 4; ++z) f[z] = (float)rec[R::HD + z];
 }
 
 // Per-cell data of one thread.
@@ -611,6 +615,26 @@ __device__ __forceinline__ void acc_moments(double* acc, const double* W, double
 // by the FP32 rounding of one increment (~1e-7), so 1/sqrt(1 + d) comes from
 // its series 1 - d/2 + 3d^2/8 - 5d^3/16 (remainder < 35 d^4/128 < 3e-17 for
 // |d| < 1e-4); a larger deviation takes sqrt and divide.
+// The FP32-state candidate: psi0 + increment in FP32, its norm and the
+// renormalisation factor in FP64 (the same series), the correction applied
+// by one FP32 FMA per component.
+__device__ __forceinline__ Sp<float> candidate_f(const Sp<float>& p0, const Sp<float>& df, int renorm) {
+    Sp<float> v{p0.ar + df.ar, p0.ai + df.ai, p0.br + df.br, p0.bi + df.bi};
+    if (renorm) {
+        const double n2 = ((double)v.ar * (double)v.ar + (double)v.ai * (double)v.ai) +
+                          ((double)v.br * (double)v.br + (double)v.bi * (double)v.bi);
+        const double d = n2 - 1.0;
+        double sc;
+        if (fabs(d) < 1e-4) sc = fma(d, fma(d, fma(d, -0.3125, 0.375), -0.5), 1.0);
+        else sc = 1.0 / sqrt(n2);
+        const float dm = (float)(sc - 1.0);
+        v.ar = fmaf(v.ar, dm, v.ar);
+        v.ai = fmaf(v.ai, dm, v.ai);
+        v.br = fmaf(v.br, dm, v.br);
+        v.bi = fmaf(v.bi, dm, v.bi);
+    }
+    return v;
+}
 __device__ __forceinline__ Sp<double> candidate(const Sp<double>& p0, const Sp<float>& df, int renorm) {
     Sp<double> nd{p0.ar + (double)df.ar, p0.ai + (double)df.ai, p0.br + (double)df.br, p0.bi + (double)df.bi};
     if (renorm) {
@@ -1498,25 +1522,29 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 }
                                 const Sp<float> u = delta_f(P, d3);
                                 const Sp<float> d3n = sp_add(d3, sp_add(delta_f(P, a), u));
-                                const double* hd = rec + R::HD;
-                                const double hv0 = te[0], hv1 = te[1];
-                                const double sg = (s & 1) ? -1.0 : 1.0;
-                                const PropF PD{0.f, (float)-fma(hd[3], hv0, sg * hd[0]), (float)hd[2],
-                                               (float)-fma(hd[3], hv1, sg * hd[1])};
+                                // -i Hd psi0 with Hd's energy-independent part combined
+                                // per beam in FP64 (the cancellation of the first-order
+                                // terms happens there) and applied in FP32
+                                const float* hdf = recf + 16;
+                                const float sg = (s & 1) ? -1.f : 1.f;
+                                const PropF PD{0.f, -fmaf(hdf[3], tef[0], sg * hdf[0]), hdf[2],
+                                               -fmaf(hdf[3], tef[1], sg * hdf[1])};
                                 Q = pf_add(pf_add(Q, PR), PD);
                                 const Sp<float> E = sp_add(delta_f(Q, a), u);
                                 df = sp_avg(d3n, d4);
                                 dsq = .25 * (double)((E.ar * E.ar + E.ai * E.ai) + (E.br * E.br + E.bi * E.bi));
                             }
-                            Sp<double> nd = candidate(sp_d(C.p0[k][s]), df, I.renorm);
-                            Sp<float> vf = sp_f(nd);
-                            src_add<CT>(te, s, vf, X0, X2, X3);
                             if constexpr (std::is_same<ST, float>::value) {
+                                const Sp<float> vf = candidate_f(a, df, I.renorm);
+                                src_add<CT>(te, s, vf, X0, X2, X3);
                                 if (!STREAM) C.k4[k][s] = vf;        // the candidate state itself
+                                if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), sp_d(vf));
                             } else {
+                                Sp<double> nd = candidate(sp_d(C.p0[k][s]), df, I.renorm);
+                                src_add<CT>(te, s, sp_f(nd), X0, X2, X3);
                                 if (!STREAM) C.k4[k][s] = df;        // increment; re-formed on accept
+                                if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), nd);
                             }
-                            if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), nd);
                         }
                         // error weights of the 4 particles (NBeam.cpp:922): with 2 stored
                         // species nu_x / anti-nu_x share |dpsi|^2 with nu_e / anti-nu_e
