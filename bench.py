@@ -55,9 +55,16 @@ FP64_PEAK_TFLOPS = 78.6
 METRIC = "beam-steps/sec (angle x energy x dr) at stated P_ee error vs FP64; % HBM roofline"
 R0, DR0 = 10.0, 1e-3
 PROF = os.path.join(ROOT, "profiles")
-PROFILE_HBM = os.path.join(PROF, "k_step_fp32_ma_1000x400_hbm.md")
-PROFILE_VALU = os.path.join(PROF, "k_step_fp32_ma_1000x400_valu.md")
-PROFILED = ("fp32", 1, 4, 4)         # precision, mode, cells per lane, beams per tile of those profiles
+# committed rocprof summaries (HBM traffic file, SQ counter file) of the
+# launch shapes they were taken at: (workload, precision) -> (mode, cells per
+# lane, beams per tile or None, block or None)
+PROFILES = {
+    ("ma1000x400", "fp32"): ((1, 4, 4, None), "k_step_fp32_ma_1000x400_hbm.md", "k_step_fp32_ma_1000x400_valu.md"),
+    ("ma1000x400", "fp64"): ((4, 1, 1, None), "k_step_fp64_ma_1000x400.md", "k_step_fp64_ma_1000x400.md"),
+    ("ma1000x400", "mixed"): ((1, 4, 5, None), "k_step_mixed_ma_1000x400.md", "k_step_mixed_ma_1000x400.md"),
+    ("maa256x64x200", "fp32"): ((4, 2, 5, None), "k_step_fp32_maa_256x64x200.md", "k_step_fp32_maa_256x64x200.md"),
+    ("ensemble96", "fp32"): ((4, 1, None, 832), "k_step_fp32_ensemble96.md", "k_step_fp32_ensemble96.md"),
+}
 ACC_WINDOW = dict(R0=300.0, R1=340.0, points=20)
 
 WORKLOADS = {
@@ -390,46 +397,63 @@ def _profile_value(path, pattern):
         return None
 
 
-def _profiled(prec_name, launch):
-    return (prec_name, launch["mode"], launch["cpt"], launch["beams_per_tile"]) == PROFILED
+def _profile(workload, prec_name, launch):
+    """(HBM file, SQ file) of the committed profiles when they were taken at
+    this launch shape, else None."""
+    ent = PROFILES.get((workload, prec_name))
+    if not ent:
+        return None
+    (mode, cpt, bpt, block), hbm, valu = ent
+    if (launch["mode"], launch["cpt"]) != (mode, cpt):
+        return None
+    if (bpt is not None and launch.get("beams_per_tile") != bpt) or (block is not None and launch.get("block") != block):
+        return None
+    return os.path.join(PROF, hbm), os.path.join(PROF, valu)
 
 
-def _roofline(prec_name, cells, steps, kernel_ms, launch):
+def _roofline(workload, prec_name, cells, steps, kernel_ms, launch):
     """HBM roofline by algorithmic bytes (the metric's figure), with the
     measured HBM traffic of the committed FETCH_SIZE / WRITE_SIZE profile when
     it covers this launch shape."""
     b = _state_bytes(prec_name)
     secs = kernel_ms / 1e3
     achieved = b * cells * steps / secs / 1e9
-    per_step = _profile_value(PROFILE_HBM, r"^traffic_bytes_per_step:\s*([0-9.eE+]+)") if _profiled(prec_name, launch) else None
+    prof = _profile(workload, prec_name, launch)
+    per_step = _profile_value(prof[0], r"^traffic_bytes_per_step:\s*([0-9.eE+]+)") if prof else None
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
            "traffic": per_step * steps if per_step else None,
-           "traffic_source": os.path.relpath(PROFILE_HBM, ROOT) if per_step else None,
+           "traffic_source": os.path.relpath(prof[0], ROOT) if per_step else None,
            "note": f"achieved = algorithmic {b} B per beam-step (nu + anti-nu 2-spinors read and written once) over "
                    f"the stepper kernel's HIP-event time (mode {launch['mode']})"}
     if per_step:
         meas = per_step * steps / secs / 1e9
         out["measured_hbm_gbs"] = meas
         out["measured_frac"] = meas / HBM_PEAK_GBS
+        out["traffic_vs_algorithmic"] = per_step / (b * cells)
         if launch["mode"] in (0, 1):
             out["note"] += ("; the resident mode keeps the state in registers across steps, so the measured HBM "
-                            "traffic (the exchange rows) is a tenth of the algorithmic bytes and HBM does not bound "
-                            "the kernel: VALU issue and the exchanges' latency do (roofline_valu)")
+                            "traffic (the exchange rows) is a small fraction of the algorithmic bytes and HBM does "
+                            "not bound the kernel: VALU issue and the exchanges' latency do (roofline_valu)")
+        else:
+            out["note"] += ("; the streaming mode reads the state in each of the step's three passes and writes it "
+                            "once (traffic_vs_algorithmic)")
     return out
 
 
-def _roofline_valu(prec_name, steps, kernel_ms, launch):
+def _roofline_valu(workload, prec_name, steps, kernel_ms, launch):
     """VALU issue rate and FP32 / FP64 FLOP rates of the timed kernel from the
     committed SQ profile of the same launch shape (instructions per wave per
     step x waves x steps over the HIP-event time)."""
-    if not _profiled(prec_name, launch):
+    prof = _profile(workload, prec_name, launch)
+    if not prof:
         return None
-    row = r"^\| {} \| [0-9,]+ \| ([0-9,]+) \|"
-    per = {k: _profile_value(PROFILE_VALU, row.format(k)) for k in (
+    row = r"^\| {} \|(?: [0-9,]+ \|)? ([0-9,.]+) \|"
+    per = {k: _profile_value(prof[1], row.format(k)) for k in (
         "SQ_INSTS_VALU", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_FMA_F32",
         "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64")}
     if per["SQ_INSTS_VALU"] is None:
         return None
+    per = {k: (v or 0.) for k, v in per.items()}        # a class the profile does not list issued none
     waves = launch["grid"] * (launch["block"] // 64)
     secs = kernel_ms / 1e3
     ach = per["SQ_INSTS_VALU"] * waves * steps / secs
@@ -441,7 +465,7 @@ def _roofline_valu(prec_name, steps, kernel_ms, launch):
             "frac": ach / VALU_PEAK_WAVE_INSTR, "valu_per_wave_step": per["SQ_INSTS_VALU"], "waves": waves,
             "fp32_tflops": f32t, "fp32_frac": f32t / FP32_PEAK_TFLOPS,
             "fp64_tflops": f64t, "fp64_frac": f64t / FP64_PEAK_TFLOPS,
-            "source": os.path.relpath(PROFILE_VALU, ROOT),
+            "source": os.path.relpath(prof[1], ROOT),
             "note": "peak: 1,024 SIMDs x one wave64 VALU instruction per 4 cycles at 2.4 GHz; FLOP rates count an "
                     "FMA as 2 against 157.3 TF FP32 and 78.6 TF FP64"}
 
@@ -473,9 +497,9 @@ def bench_ma(a, prec_name):
         "config": {"workload": w["desc"], "beams": cells, "precision": prec_name, "launch": res["launch"]},
         "accepted_steps": res["accepted"], "radius_km": res["r"],
         "kernel_ms_per_step": res["kernel_ms"] / max(1, res["steps"]),
-        "roofline": _roofline(prec_name, cells, res["steps"], res["kernel_ms"], res["launch"]),
+        "roofline": _roofline("ma1000x400", prec_name, cells, res["steps"], res["kernel_ms"], res["launch"]),
     })
-    rv = _roofline_valu(prec_name, res["steps"], res["kernel_ms"], res["launch"])
+    rv = _roofline_valu("ma1000x400", prec_name, res["steps"], res["kernel_ms"], res["launch"])
     if rv:
         out["roofline_valu"] = rv
     if a.no_fp64 or prec_name == "fp64":
@@ -483,7 +507,8 @@ def bench_ma(a, prec_name):
     ref = time_ctx(p, "fp64", psi0, a.steps, a.warmup, a.tune_steps)
     out["fp64"] = {"value": cells * ref["steps"] / ref["wall_s"], "ms_per_step": 1e3 * ref["wall_s"] / max(1, ref["steps"]),
                    "accepted_steps": ref["accepted"], "radius_km": ref["r"], "launch": ref["launch"],
-                   "roofline": _roofline("fp64", cells, ref["steps"], ref["kernel_ms"], ref["launch"])}
+                   "roofline": _roofline("ma1000x400", "fp64", cells, ref["steps"], ref["kernel_ms"], ref["launch"]),
+                   "roofline_valu": _roofline_valu("ma1000x400", "fp64", ref["steps"], ref["kernel_ms"], ref["launch"])}
     log(f"fp64: {out['fp64']['value']:.3e} beam-steps/s")
     out["radius_per_attempt_vs_fp64"] = (res["r"] - R0) / max(ref["r"] - R0, 1e-300)
     out["pee_error_vs_fp64"] = equal_radius_errors(p, psi0, res)
@@ -530,13 +555,16 @@ def bench_maa(a, prec_name):
         p = _cfg(**dict(w["params"], Lve=lve))
         cells = p.abins * p.pbins * p.ebins
         psi0 = azimuthal_psi(p.abins, p.pbins, p.ebins, seed=31, amp=1e-7)
-        log(f"{prec_name} MAA 256x64x200 Lve = {lve}: autotune, warmup {a.warmup}, timing {a.steps} steps")
-        res = time_ctx(p, prec_name, psi0, a.steps, a.warmup, a.tune_steps)
+        launch = tuple(int(x) for x in a.launch.split(",")) if a.launch else None
+        log(f"{prec_name} MAA 256x64x200 Lve = {lve}: {'launch ' + a.launch if launch else 'autotune'}, "
+            f"warmup {a.warmup}, timing {a.steps} steps")
+        res = time_ctx(p, prec_name, psi0, a.steps, a.warmup, a.tune_steps, launch)
         pt = {"Lve_erg_s": float(lve), "value": cells * res["steps"] / res["wall_s"],
               "ms_per_step": 1e3 * res["wall_s"] / max(1, res["steps"]),
               "kernel_ms_per_step": res["kernel_ms"] / max(1, res["steps"]),
               "accepted_steps": res["accepted"], "radius_km": res["r"], "launch": res["launch"],
-              "roofline": _roofline(prec_name, cells, res["steps"], res["kernel_ms"], res["launch"])}
+              "roofline": _roofline("maa256x64x200", prec_name, cells, res["steps"], res["kernel_ms"], res["launch"]),
+              "roofline_valu": _roofline_valu("maa256x64x200", prec_name, res["steps"], res["kernel_ms"], res["launch"])}
         if not a.no_fp64 and prec_name != "fp64":
             ref = time_ctx(p, "fp64", psi0, a.steps, a.warmup, a.tune_steps)
             pt["fp64"] = {"value": cells * ref["steps"] / ref["wall_s"],
@@ -579,9 +607,11 @@ def bench_ensemble(a, prec_name):
                    "launch": res["launch"]},
         "kernel_ms_per_step": res["kernel_ms"] / max(1, max(res["steps"])),
         "accepted_steps_total": int(sum(res["accepted"])),
-        "roofline": _roofline(prec_name, cells, max(res["steps"]), res["kernel_ms"],
-                              dict(res["launch"], beams_per_tile=0)),
+        "roofline": _roofline("ensemble96", prec_name, cells, max(res["steps"]), res["kernel_ms"], res["launch"]),
     })
+    rv = _roofline_valu("ensemble96", prec_name, max(res["steps"]), res["kernel_ms"], res["launch"])
+    if rv:
+        out["roofline_valu"] = rv
     log(f"{prec_name}: {value:.3e} beam-steps/s, launch {res['launch']}")
     if not a.no_fp64 and prec_name != "fp64":
         ens64, ref = time_ensemble(params, "fp64", inits, a.steps, a.warmup, a.tune_steps)
@@ -656,7 +686,8 @@ def main():
                     help="headline only: no normal hierarchy, short launches, mixed leg or other workloads")
     ap.add_argument("--cpu-cells", type=float, default=3.2e8, help="beam-steps of each CPU-baseline sample at 16 threads")
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--launch", default=None, help="mode,cpt,bpt: fixed launch shape instead of autotuning")
+    ap.add_argument("--launch", default=None,
+                    help="mode,cpt,bpt: fixed launch shape instead of autotuning (MA and MAA workloads)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (several GPUs would shard the "
