@@ -39,12 +39,13 @@
 // Taylor series in x^2 below x^2 = 1/16 (to x^16, and to x^8 / x^9 below
 // x^2 = 2^-10; prop_ds) and from sincos above; where one Hamiltonian acts
 // over dl and 2 dl the longer step comes from the double-angle identities
-// (prop_d2).  The FP32 modes propagate increments delta = (U - I) psi in
-// FP32, with cos x - 1 and sin(x)/x - 1 as polynomials in x^2 (prop_f2s: a
+// (prop_d2).  The FP32 modes apply U - I in FP32 by FMA chains (delta_acc),
+// with cos x - 1 and sin(x)/x - 1 as polynomials in x^2 (prop_f2s: a
 // wave-uniform linear tier when every lane has x^2 < 2^-10, Chebyshev
 // cubics below x^2 = 1, hardware sin/cos of the half angle above), and
-// estimate the step error from second-order remainders plus an FP64
-// first-order term (pass C); moments, error sums and the decision are FP64.
+// estimate the step error from second-order remainders plus the first-order
+// term -i Hd psi0 with Hd combined per beam in FP64 (pass C); moments, error
+// sums and the decision are FP64.
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include <cstdint>
@@ -206,12 +207,6 @@ __device__ __forceinline__ void prop_f2s(float h0, float h1, float h2, float dl,
 __device__ __forceinline__ PropF pf_of(float h0, float h1, float h2, float cm1, float c) {
     return PropF{cm1, -h0 * c, h2 * c, -h1 * c};
 }
-__device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, PropF& P, PropF& PR) {
-    float cm1, sc, scr;
-    prop_f2s(h0, h1, h2, dl, cm1, sc, scr);
-    P = pf_of(h0, h1, h2, cm1, sc);
-    PR = pf_of(h0, h1, h2, cm1, scr);
-}
 // Double angle.  Where one Hamiltonian acts over dl and over 2 dl (the first
 // two sub-steps of a step of the multi-angle and single-angle geometries,
 // dr / c0 and (dr / 2) / c0, which differ by exactly a factor of 2),
@@ -225,24 +220,30 @@ __device__ __forceinline__ void prop_f2(float h0, float h1, float h2, float dl, 
 __device__ __forceinline__ PropF pf_rdiff2(float h0, float h1, float h2, float cm1, float sc, float scr) {
     return pf_of(h0, h1, h2, -cm1 * fmaf(2.f, cm1, 3.f), -fmaf(2.f * sc, cm1, scr));
 }
-__device__ __forceinline__ Sp<float> delta_f(const PropF& P, const Sp<float>& a) {
+// c + (U - I) a with one FMA chain per component: the four products are
+// accumulated onto c (four instructions a component, where the paired
+// products and their sum take five).  With c = a this is U a itself; with
+// c = 0 the increment.
+__device__ __forceinline__ Sp<float> delta_acc(const PropF& P, const Sp<float>& a, const Sp<float>& c) {
     Sp<float> o;
-    o.ar = (P.cm1 * a.ar - P.m1 * a.ai) + (P.m2 * a.br - P.m3 * a.bi);
-    o.ai = (P.cm1 * a.ai + P.m1 * a.ar) + (P.m2 * a.bi + P.m3 * a.br);
-    o.br = (-P.m2 * a.ar - P.m3 * a.ai) + (P.cm1 * a.br + P.m1 * a.bi);
-    o.bi = (-P.m2 * a.ai + P.m3 * a.ar) + (P.cm1 * a.bi - P.m1 * a.br);
+    o.ar = fmaf(P.cm1, a.ar, fmaf(-P.m1, a.ai, fmaf(P.m2, a.br, fmaf(-P.m3, a.bi, c.ar))));
+    o.ai = fmaf(P.cm1, a.ai, fmaf(P.m1, a.ar, fmaf(P.m2, a.bi, fmaf(P.m3, a.br, c.ai))));
+    o.br = fmaf(-P.m2, a.ar, fmaf(-P.m3, a.ai, fmaf(P.cm1, a.br, fmaf(P.m1, a.bi, c.br))));
+    o.bi = fmaf(-P.m2, a.ai, fmaf(P.m3, a.ar, fmaf(P.cm1, a.bi, fmaf(-P.m1, a.br, c.bi))));
     return o;
+}
+// q + sg (cm1, -i scr H): a remainder's coefficient set (pf_of(h, cm1, scr))
+// added (sg = 1) or subtracted (sg = -1) with one FMA per component instead
+// of its three products and four sums
+template <int SG>
+__device__ __forceinline__ PropF pf_fma(const PropF& q, float h0, float h1, float h2, float cm1, float scr) {
+    if constexpr (SG > 0) return PropF{q.cm1 + cm1, fmaf(-h0, scr, q.m1), fmaf(h2, scr, q.m2), fmaf(-h1, scr, q.m3)};
+    else return PropF{q.cm1 - cm1, fmaf(h0, scr, q.m1), fmaf(-h2, scr, q.m2), fmaf(h1, scr, q.m3)};
 }
 // (U - I) is linear in its coefficients: sums of coefficient sets apply as one.
 __device__ __forceinline__ PropF pf_add(const PropF& p, const PropF& q) {
     return PropF{p.cm1 + q.cm1, p.m1 + q.m1, p.m2 + q.m2, p.m3 + q.m3};
 }
-__device__ __forceinline__ PropF pf_sub(const PropF& p, const PropF& q) {
-    return PropF{p.cm1 - q.cm1, p.m1 - q.m1, p.m2 - q.m2, p.m3 - q.m3};
-}
-// coefficient sets kept in an Sp<float> register slot between passes
-__device__ __forceinline__ Sp<float> pf_pack(const PropF& p) { return Sp<float>{p.cm1, p.m1, p.m2, p.m3}; }
-__device__ __forceinline__ PropF pf_unpack(const Sp<float>& v) { return PropF{v.ar, v.ai, v.br, v.bi}; }
 template <class T> __device__ __forceinline__ Sp<T> sp_add(const Sp<T>& a, const Sp<T>& b) {
     return Sp<T>{a.ar + b.ar, a.ai + b.ai, a.br + b.br, a.bi + b.bi};
 }
@@ -261,7 +262,10 @@ __device__ __forceinline__ Sp<double> sp_d(const Sp<double>& a) { return a; }
 
 // density matrix components 0, 2, 3 (NBeam.cpp:314-322)
 template <class T> __device__ __forceinline__ void dens(const Sp<T>& a, T& d0, T& d2, T& d3) {
-    d0 = T(.5) * ((a.ar * a.ar + a.ai * a.ai) - (a.br * a.br + a.bi * a.bi));
+    if constexpr (std::is_same<T, float>::value)
+        d0 = .5f * fmaf(a.ar, a.ar, fmaf(a.ai, a.ai, -fmaf(a.br, a.br, a.bi * a.bi)));
+    else
+        d0 = T(.5) * ((a.ar * a.ar + a.ai * a.ai) - (a.br * a.br + a.bi * a.bi));
     d2 = (a.ar * a.br) - (a.ai * (-a.bi));
     d3 = (a.ar * (-a.bi)) + (a.ai * a.br);
 }
@@ -615,11 +619,11 @@ __device__ __forceinline__ void acc_moments(double* acc, const double* W, double
 // by the FP32 rounding of one increment (~1e-7), so 1/sqrt(1 + d) comes from
 // its series 1 - d/2 + 3d^2/8 - 5d^3/16 (remainder < 35 d^4/128 < 3e-17 for
 // |d| < 1e-4); a larger deviation takes sqrt and divide.
-// The FP32-state candidate: psi0 + increment in FP32, its norm and the
-// renormalisation factor in FP64 (the same series), the correction applied
-// by one FP32 FMA per component.
-__device__ __forceinline__ Sp<float> candidate_f(const Sp<float>& p0, const Sp<float>& df, int renorm) {
-    Sp<float> v{p0.ar + df.ar, p0.ai + df.ai, p0.br + df.br, p0.bi + df.bi};
+// The FP32-state candidate: psi0 + t / 2 with t = d3n + d4 (one FMA a
+// component), its norm and the renormalisation factor in FP64 (the same
+// series), the correction applied by one FP32 FMA per component.
+__device__ __forceinline__ Sp<float> candidate_t(const Sp<float>& p0, const Sp<float>& t, int renorm) {
+    Sp<float> v{fmaf(.5f, t.ar, p0.ar), fmaf(.5f, t.ai, p0.ai), fmaf(.5f, t.br, p0.br), fmaf(.5f, t.bi, p0.bi)};
     if (renorm) {
         const double n2 = ((double)v.ar * (double)v.ar + (double)v.ai * (double)v.ai) +
                           ((double)v.br * (double)v.br + (double)v.bi * (double)v.bi);
@@ -1234,42 +1238,39 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             src_add<CT>(te, s, v1, X0, X2, X3);
                             src_add<CT>(te, s, v3, Y0, Y2, Y3);
                         } else if constexpr (DBL) {
-                            // dl1 = 2 dl0: U(dl1) - I = 2 D0 + D0^2
+                            // dl1 = 2 dl0: U(dl1) = U(dl0)^2, so only the states' densities
+                            // need psi3 = U0 psi0 and psi1 = U0 psi3
                             float f0, f1, f2;
                             cell_hf(tef, s, recf + 0, f0, f1, f2);
                             float cm1, sc, scr;
                             prop_f2s(f0, f1, f2, recf[12], cm1, sc, scr);
                             const PropF P0 = pf_of(f0, f1, f2, cm1, sc);
-                            Sp<float> a = sp_f(C.p0[k][s]);
-                            Sp<float> d3 = delta_f(P0, a);
-                            Sp<float> d1 = sp_add(sp_add(d3, d3), delta_f(P0, d3));
+                            const Sp<float> a = sp_f(C.p0[k][s]);
+                            const Sp<float> p3 = delta_acc(P0, a, a);
+                            src_add<CT>(te, s, delta_acc(P0, p3, p3), X0, X2, X3);
+                            src_add<CT>(te, s, p3, Y0, Y2, Y3);
                             if (MODE == 0) {
-                                // kept for pass C: the remainder coefficients Q = R0 - R1
-                                // so far, and delta3
+                                // kept for pass C, formed as pass C forms them in the
+                                // other modes: Q = R0 - R1 so far, and delta3
                                 const PropF q = pf_rdiff2(f0, f1, f2, cm1, sc, scr);
                                 s_keep[(k * NS + s) * B + tid] = make_float4(q.cm1, q.m1, q.m2, q.m3);
-                                C.k3[k][s] = d3;
+                                C.k3[k][s] = delta_acc(P0, a, Sp<float>{-0.f, -0.f, -0.f, -0.f});
                             }
-                            src_add<CT>(te, s, sp_add(a, d1), X0, X2, X3);
-                            src_add<CT>(te, s, sp_add(a, d3), Y0, Y2, Y3);
                         } else {
-                            PropF P1, P0, PR1, PR0;
-                            float f0, f1, f2;
+                            float f0, f1, f2, c1, s1, r1, c0, s0, r0;
                             cell_hf(tef, s, recf + 0, f0, f1, f2);
-                            prop_f2(f0, f1, f2, recf[13], P1, PR1);
-                            prop_f2(f0, f1, f2, recf[12], P0, PR0);
-                            Sp<float> a = sp_f(C.p0[k][s]);
-                            Sp<float> d1 = delta_f(P1, a), d3 = delta_f(P0, a);
+                            prop_f2s(f0, f1, f2, recf[13], c1, s1, r1);
+                            prop_f2s(f0, f1, f2, recf[12], c0, s0, r0);
+                            const PropF P0 = pf_of(f0, f1, f2, c0, s0);
+                            const Sp<float> a = sp_f(C.p0[k][s]);
+                            src_add<CT>(te, s, delta_acc(pf_of(f0, f1, f2, c1, s1), a, a), X0, X2, X3);
+                            src_add<CT>(te, s, delta_acc(P0, a, a), Y0, Y2, Y3);
                             if (MODE == 0) {
-                                // kept for pass C: the remainder coefficients Q = R0 - R1
-                                // so far, and delta3
-                                const PropF Q{0.f - PR1.cm1, 0.f - PR1.m1, 0.f - PR1.m2, 0.f - PR1.m3};
-                                const PropF q = pf_add(Q, PR0);
+                                const PropF q = pf_fma<1>(pf_fma<-1>(PropF{-0.f, -0.f, -0.f, -0.f}, f0, f1, f2, c1, r1),
+                                                          f0, f1, f2, c0, r0);
                                 s_keep[(k * NS + s) * B + tid] = make_float4(q.cm1, q.m1, q.m2, q.m3);
-                                C.k3[k][s] = d3;
+                                C.k3[k][s] = delta_acc(P0, a, Sp<float>{-0.f, -0.f, -0.f, -0.f});
                             }
-                            src_add<CT>(te, s, sp_add(a, d1), X0, X2, X3);
-                            src_add<CT>(te, s, sp_add(a, d3), Y0, Y2, Y3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);        // h1 at r+dr
@@ -1346,21 +1347,23 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             src_add<CT>(te, s, v4, X0, X2, X3);
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
-                            PropF Pb, P4, PRb, PR4;
-                            float f0, f1, f2;
+                            float f0, f1, f2, cm1, sc, scr;
                             cell_hf(tef, s, recf + 6, f0, f1, f2);
-                            prop_f2(f0, f1, f2, recf[13], P4, PR4);
-                            Sp<float> d4 = delta_f(P4, a);
+                            prop_f2s(f0, f1, f2, recf[13], cm1, sc, scr);
+                            const PropF P4 = pf_of(f0, f1, f2, cm1, sc);
+                            src_add<CT>(te, s, delta_acc(P4, a, a), X0, X2, X3);    // psi4 = U4 psi0
                             if (MODE == 0) {
-                                // Q -= Rb, Q += R3 (the order of the recompute path)
-                                cell_hf(tef, s, recf + 3, f0, f1, f2);
-                                prop_f2(f0, f1, f2, recf[13], Pb, PRb);
+                                // Q -= Rb, Q += R3 and delta4, as pass C forms them in the
+                                // other modes
+                                float g0, g1, g2, gc, gs, gr;
+                                cell_hf(tef, s, recf + 3, g0, g1, g2);
+                                prop_f2s(g0, g1, g2, recf[13], gc, gs, gr);
                                 const float4 o = s_keep[(k * NS + s) * B + tid];
-                                const PropF q = pf_add(pf_sub(PropF{o.x, o.y, o.z, o.w}, PRb), PR4);
+                                const PropF q = pf_fma<1>(pf_fma<-1>(PropF{o.x, o.y, o.z, o.w}, g0, g1, g2, gc, gr),
+                                                          f0, f1, f2, cm1, scr);
                                 s_keep[(k * NS + s) * B + tid] = make_float4(q.cm1, q.m1, q.m2, q.m3);
-                                C.k4[k][s] = d4;
+                                C.k4[k][s] = delta_acc(P4, a, Sp<float>{-0.f, -0.f, -0.f, -0.f});
                             }
-                            src_add<CT>(te, s, sp_add(a, d4), X0, X2, X3);
                         }
                     }
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
@@ -1471,7 +1474,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                             if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), fin);
                         } else {
                             Sp<float> a = sp_f(C.p0[k][s]);
-                            Sp<float> d3, d4, df;
+                            Sp<float> d3, d4, dn;
                             static_assert(COMP, "FP32 increments always use the compensated error estimate");
                             {
                                 // increments of the state as below; the error from the
@@ -1486,60 +1489,68 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                 // P4 d3 shared with the state's d3n.  Mode 0 kept
                                 // Q = R0 - R1 - Rb + R3, d3 and d4 from passes A and
                                 // B; modes 1/2 recompute them here in the same order.
-                                PropF P, PR, Q;
+                                // Remainder sets are accumulated into Q by one FMA per
+                                // component (pf_fma), increments by FMA chains
+                                // (delta_acc).
+                                const Sp<float> Z{-0.f, -0.f, -0.f, -0.f};
+                                PropF Q;
+                                // h3's propagator (modes 1/2): its increment d4 = D3 psi0
+                                // is only needed in d3n + d4 = d3 + u + (D3 + D4) psi0,
+                                // so its coefficient set is added to h4's and applied once
+                                float k0 = 0.f, k1 = 0.f, k2 = 0.f, kc = 0.f, ks = 0.f;
                                 if (MODE == 0) {
                                     const float4 o = s_keep[(k * NS + s) * B + tid];
                                     Q = PropF{o.x, o.y, o.z, o.w};
                                     d3 = C.k3[k][s];
                                     d4 = C.k4[k][s];
                                 } else {
-                                float f0, f1, f2;
+                                float f0, f1, f2, cm1, sc, scr;
                                 cell_hf(tef, s, recf + 0, f0, f1, f2);                         // h0
                                 if constexpr (DBL) {
-                                    float cm1, sc, scr;
                                     prop_f2s(f0, f1, f2, recf[12], cm1, sc, scr);
-                                    d3 = delta_f(pf_of(f0, f1, f2, cm1, sc), a);
+                                    d3 = delta_acc(pf_of(f0, f1, f2, cm1, sc), a, Z);
                                     Q = pf_rdiff2(f0, f1, f2, cm1, sc, scr);
                                 } else {
-                                prop_f2(f0, f1, f2, recf[13], P, PR);
-                                Q = PropF{0.f - PR.cm1, 0.f - PR.m1, 0.f - PR.m2, 0.f - PR.m3};
-                                prop_f2(f0, f1, f2, recf[12], P, PR);
-                                d3 = delta_f(P, a);
-                                Q = pf_add(Q, PR);
+                                    prop_f2s(f0, f1, f2, recf[13], cm1, sc, scr);
+                                    Q = pf_fma<-1>(PropF{-0.f, -0.f, -0.f, -0.f}, f0, f1, f2, cm1, scr);
+                                    prop_f2s(f0, f1, f2, recf[12], cm1, sc, scr);
+                                    d3 = delta_acc(pf_of(f0, f1, f2, cm1, sc), a, Z);
+                                    Q = pf_fma<1>(Q, f0, f1, f2, cm1, scr);
                                 }
                                 cell_hf(tef, s, recf + 3, f0, f1, f2);                         // h1
-                                prop_f2(f0, f1, f2, recf[13], P, PR);
-                                Q = pf_sub(Q, PR);
+                                prop_f2s(f0, f1, f2, recf[13], cm1, sc, scr);
+                                Q = pf_fma<-1>(Q, f0, f1, f2, cm1, scr);
                                 cell_hf(tef, s, recf + 6, f0, f1, f2);                         // h3
-                                prop_f2(f0, f1, f2, recf[13], P, PR);
-                                d4 = delta_f(P, a);
-                                Q = pf_add(Q, PR);
+                                prop_f2s(f0, f1, f2, recf[13], cm1, sc, scr);
+                                k0 = f0; k1 = f1; k2 = f2; kc = cm1; ks = sc;
+                                Q = pf_fma<1>(Q, f0, f1, f2, cm1, scr);
                                 }
-                                {
-                                    float f0, f1, f2;
-                                    cell_hf(tef, s, recf + 9, f0, f1, f2);
-                                    prop_f2(f0, f1, f2, recf[14], P, PR);                      // h4
-                                }
-                                const Sp<float> u = delta_f(P, d3);
-                                const Sp<float> d3n = sp_add(d3, sp_add(delta_f(P, a), u));
+                                float g0, g1, g2, gc, gs, gr;
+                                cell_hf(tef, s, recf + 9, g0, g1, g2);
+                                prop_f2s(g0, g1, g2, recf[14], gc, gs, gr);                    // h4
+                                const PropF P = pf_of(g0, g1, g2, gc, gs);
+                                const Sp<float> u = delta_acc(P, d3, Z);
+                                // dn = d3n + d4 with d3n = d3 + D4 (psi0 + d3) = d3 + u + D4 psi0
+                                if (MODE == 0) dn = delta_acc(P, a, sp_add(sp_add(d3, u), d4));
+                                else dn = delta_acc(pf_fma<1>(P, k0, k1, k2, kc, ks), a, sp_add(d3, u));
                                 // -i Hd psi0 with Hd's energy-independent part combined
                                 // per beam in FP64 (the cancellation of the first-order
                                 // terms happens there) and applied in FP32
                                 const float* hdf = recf + 16;
                                 const float sg = (s & 1) ? -1.f : 1.f;
-                                const PropF PD{0.f, -fmaf(hdf[3], tef[0], sg * hdf[0]), hdf[2],
+                                const PropF PD{-0.f, -fmaf(hdf[3], tef[0], sg * hdf[0]), hdf[2],
                                                -fmaf(hdf[3], tef[1], sg * hdf[1])};
-                                Q = pf_add(pf_add(Q, PR), PD);
-                                const Sp<float> E = sp_add(delta_f(Q, a), u);
-                                df = sp_avg(d3n, d4);
-                                dsq = .25 * (double)((E.ar * E.ar + E.ai * E.ai) + (E.br * E.br + E.bi * E.bi));
+                                Q = pf_add(pf_fma<1>(Q, g0, g1, g2, gc, gr), PD);
+                                const Sp<float> E = delta_acc(Q, a, u);
+                                dsq = .25 * (double)fmaf(E.ar, E.ar, fmaf(E.ai, E.ai, fmaf(E.br, E.br, E.bi * E.bi)));
                             }
                             if constexpr (std::is_same<ST, float>::value) {
-                                const Sp<float> vf = candidate_f(a, df, I.renorm);
+                                const Sp<float> vf = candidate_t(a, dn, I.renorm);
                                 src_add<CT>(te, s, vf, X0, X2, X3);
                                 if (!STREAM) C.k4[k][s] = vf;        // the candidate state itself
                                 if (STREAM) st_sp<ST>((ST*)I.psi[cur ^ 1], ncells, s, cell_index(t, k), sp_d(vf));
                             } else {
+                                const Sp<float> df{.5f * dn.ar, .5f * dn.ai, .5f * dn.br, .5f * dn.bi};
                                 Sp<double> nd = candidate(sp_d(C.p0[k][s]), df, I.renorm);
                                 src_add<CT>(te, s, sp_f(nd), X0, X2, X3);
                                 if (!STREAM) C.k4[k][s] = df;        // increment; re-formed on accept
