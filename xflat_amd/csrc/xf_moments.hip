@@ -39,14 +39,7 @@ __global__ void __launch_bounds__(MOM_BLOCK) k_moments_part(const XfInstDev* __r
     for (int e = tid; e < ne; e += MOM_BLOCK) {
         double* t = s_tab + e * TS;
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
-            if constexpr (F64) {
-                t[6 + q] = I.srcw[q * ne + e];
-            } else {
-                t[6 + q] = 0.;
-                *reinterpret_cast<float*>(t + 6 + q) = I.srcwf[q * ne + e];
-            }
-        }
+        for (int q = 0; q < NS; ++q) put_src_weight<F64>(t + 6 + q, I.srcw[q * ne + e], I.srcwf[q * ne + e]);
     }
     const int bpw = (nbeam + (int)gridDim.x - 1) / (int)gridDim.x;
     const int b0 = min(nbeam, (int)blockIdx.x * bpw), b1 = min(nbeam, b0 + bpw);

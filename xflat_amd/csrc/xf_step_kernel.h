@@ -262,10 +262,7 @@ __device__ __forceinline__ Sp<double> sp_d(const Sp<double>& a) { return a; }
 
 // density matrix components 0, 2, 3 (NBeam.cpp:314-322)
 template <class T> __device__ __forceinline__ void dens(const Sp<T>& a, T& d0, T& d2, T& d3) {
-    if constexpr (std::is_same<T, float>::value)
-        d0 = .5f * fmaf(a.ar, a.ar, fmaf(a.ai, a.ai, -fmaf(a.br, a.br, a.bi * a.bi)));
-    else
-        d0 = T(.5) * ((a.ar * a.ar + a.ai * a.ai) - (a.br * a.br + a.bi * a.bi));
+    d0 = T(.5) * ((a.ar * a.ar + a.ai * a.ai) - (a.br * a.br + a.bi * a.bi));
     d2 = (a.ar * a.br) - (a.ai * (-a.bi));
     d3 = (a.ar * (-a.bi)) + (a.ai * a.br);
 }
@@ -504,6 +501,42 @@ __device__ __forceinline__ void beam_h(double* rec, int pt, int slot, const doub
 // psi4 - psi1 (see the compensated error of the mixed stepper): per beam the
 // part of sum_k c_k dl_k H_k that does not depend on the energy, and the
 // coefficient sum_k c_k dl_k of the vacuum part.  FP64, after h4 is known.
+// Pass A's set-up in one phase for the geometries whose path lengths come
+// from single points (multi-angle: dr/2 / c0, dr / c0, dr/2 / c1; single
+// angle: the sub-step lengths): the thread of point pt forms the point, then
+// the path lengths that divide by its own cos(theta_R) and, at point 0, the
+// Hamiltonian h0 -- the same operations as beam_point, beam_dl and beam_h,
+// without a barrier between them.
+template <int GEOM>
+__device__ __forceinline__ void beam_point_a(const XfInstDev& I, int b, int pt, double rp, double* rec, double dr,
+                                             double dr2, const double* M, double coef, double hm) {
+    static_assert(GEOM != XF_MAA, "MAA path lengths average two points");
+    using R = BRec<GEOM>;
+    beam_point<GEOM>(I, b, pt, rp, rec);
+    float* f = reinterpret_cast<float*>(rec + R::HF) + 12;
+    if constexpr (GEOM == XF_SA) {
+        if (pt == 0) {
+            rec[R::DL + 0] = dr2;
+            rec[R::DL + 1] = dr;
+            rec[R::DL + 2] = dr2;
+            f[0] = (float)dr2;
+            f[1] = (float)dr;
+            f[2] = (float)dr2;
+        }
+    } else {
+        const double c = rec[R::G + pt * 3];
+        if (pt == 0) {
+            rec[R::DL + 0] = dr2 / c;
+            rec[R::DL + 1] = dr / c;
+            f[0] = (float)rec[R::DL + 0];
+            f[1] = (float)rec[R::DL + 1];
+        } else if (pt == 1) {
+            rec[R::DL + 2] = dr2 / c;
+            f[2] = (float)rec[R::DL + 2];
+        }
+    }
+    if (pt == 0) beam_h<GEOM>(rec, 0, 0, M, coef, hm);
+}
 template <int GEOM>
 __device__ __forceinline__ void beam_hd(double* rec) {
     using R = BRec<GEOM>;
@@ -591,15 +624,37 @@ __device__ __forceinline__ void cell_hf(const float* tef, int s, const float* Hf
 // Source contribution of one cell to the angular integral, X(z) for density
 // components 0, 2, 3 (upd_nu_coef, NBeam.h:47-53, with L/<E> folded into the
 // per-energy source weights).
+// The per-energy source weight slot of one species: the FP64 weight, or for
+// FP32 cells the FP32 weight and its half (src_add).
+template <bool F64>
+__device__ __forceinline__ void put_src_weight(double* slot, double w, float wf) {
+    if constexpr (F64) {
+        *slot = w;
+    } else {
+        float* f = reinterpret_cast<float*>(slot);
+        f[0] = wf;
+        f[1] = .5f * wf;
+    }
+}
+// FP32: the slot holds the weight w and w / 2, which takes the density's
+// factor 1/2 (d0 = (|a|^2 - |b|^2) / 2 by an FMA chain; w (t / 2) = (w / 2) t
+// exactly).
 template <class CT>
 __device__ __forceinline__ void src_add(const double* te, int s, const Sp<CT>& v, CT& X0, CT& X2, CT& X3) {
-    CT d0, d2, d3;
-    dens<CT>(v, d0, d2, d3);
-    CT w;
-    if constexpr (std::is_same<CT, double>::value) w = te[6 + s];
-    else w = *reinterpret_cast<const float*>(te + 6 + s);
-    if (s & 1) { X0 -= w * d0; X2 -= w * d2; }
-    else       { X0 += w * d0; X2 += w * d2; }
+    CT d0, d2, d3, w, w0;
+    if constexpr (std::is_same<CT, double>::value) {
+        dens<CT>(v, d0, d2, d3);
+        w = w0 = te[6 + s];
+    } else {
+        d0 = fmaf(v.ar, v.ar, fmaf(v.ai, v.ai, -fmaf(v.br, v.br, v.bi * v.bi)));
+        d2 = (v.ar * v.br) - (v.ai * (-v.bi));
+        d3 = (v.ar * (-v.bi)) + (v.ai * v.br);
+        const float* wf = reinterpret_cast<const float*>(te + 6 + s);
+        w = wf[0];
+        w0 = wf[1];
+    }
+    if (s & 1) { X0 -= w0 * d0; X2 -= w * d2; }
+    else       { X0 += w0 * d0; X2 += w * d2; }
     X3 += w * d3;
 }
 
@@ -657,17 +712,20 @@ __device__ __forceinline__ Sp<double> candidate(const Sp<double>& p0, const Sp<f
 // 64 lanes in lane 63: rotations by 8, 4, 2, 1 within each row of 16 lanes
 // give every lane its row's total, then row 0's and row 2's totals (lanes 15
 // and 47) are broadcast into rows 1 and 3, and lane 31's into rows 2 and 3.
-// The association order is fixed, so equal inputs give equal bits.
+// The association order is fixed, so equal inputs give equal bits.  The
+// moves leave lanes outside a broadcast's row mask unwritten (no zero
+// operand is set up first); those lanes never feed lane 63, the only lane
+// whose result is used.
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ double dpp_d(double v) {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWMASK, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROWMASK, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, ROWMASK, 0xF, false);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 template <bool MAX>
 __device__ __forceinline__ double dpp_op(double a, double b) { return MAX ? fmax(a, b) : a + b; }
-// MAX: maxima of values >= 0 (lanes outside a broadcast's row mask read 0).
+// MAX: maxima of values >= 0.
 template <bool MAX>
 __device__ __forceinline__ double wave_red_dpp(double v) {
     v = dpp_op<MAX>(v, dpp_d<0x128, 0xF>(v));    // row_ror:8
@@ -1000,22 +1058,16 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         double* t = s_tab + e * TS;
         t[0] = I.hv[e];
         t[1] = I.hv[ne + e];
+        // FP32 increments: the error weights carry the 1/4 of |psi4 - psi1|^2 = |E|^2 / 4
 #pragma unroll
-        for (int q = 0; q < 4; ++q) t[2 + q] = I.errw[q * ne + e];
+        for (int q = 0; q < 4; ++q) t[2 + q] = F64 ? I.errw[q * ne + e] : .25 * I.errw[q * ne + e];
         if constexpr (!F64) {
             float* tf = reinterpret_cast<float*>(t + 6 + NS);
             tf[0] = (float)I.hv[e];
             tf[1] = (float)I.hv[ne + e];
         }
 #pragma unroll
-        for (int q = 0; q < NS; ++q) {
-            if constexpr (F64) {
-                t[6 + q] = I.srcw[q * ne + e];
-            } else {
-                t[6 + q] = 0.;
-                *reinterpret_cast<float*>(t + 6 + q) = I.srcwf[q * ne + e];
-            }
-        }
+        for (int q = 0; q < NS; ++q) put_src_weight<F64>(t + 6 + q, I.srcw[q * ne + e], I.srcwf[q * ne + e]);
     }
     __syncthreads();
     __shared__ int s_flag[2];
@@ -1086,6 +1138,19 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
     };
     auto tile_dl = [&](int nbt, double ddr, double dr2) __attribute__((always_inline)) {
         for (int b = tid; b < nbt; b += B) beam_dl<GEOM>(s_beam + b * R::SIZE, ddr, dr2);
+    };
+    // pass A's points, path lengths and h0 in one phase (multi-angle and
+    // single-angle geometries)
+    auto tile_setup_a = [&](int t, int nbt, double rr, double rdr2, double rdr, double ddr, double dr2,
+                            const double* M, double hm) __attribute__((always_inline)) {
+        if constexpr (DBL) {
+            const double coef = F64 ? I.coef : I.coef_f;
+            for (int q = tid; q < 3 * nbt; q += B) {
+                const int b = q / 3, pt = q - 3 * b;
+                beam_point_a<GEOM>(I, t * bpt + b, pt, pt == 0 ? rr : (pt == 1 ? rdr2 : rdr), s_beam + b * R::SIZE,
+                                   ddr, dr2, M, coef, hm);
+            }
+        }
     };
     // mode 4: the same over every beam of the workgroup's tiles (slot q: tile
     // lb + (q / bpt) nblk, beam q % bpt of it)
@@ -1204,10 +1269,14 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     // last exchange ended with a barrier after every read of it)
                     if (MODE == 2) __syncthreads();
                     if (MODE == 2) load_p0(t, cur);      // in flight during the set-up
-                    tile_points(t, nbt, r, rdr2, rdr);
-                    __syncthreads();
-                    tile_dl(nbt, dr, dr2);
-                    tile_h(nbt, 0, 0, M0, hm_r);
+                    if constexpr (DBL) {
+                        tile_setup_a(t, nbt, r, rdr2, rdr, dr, dr2, M0, hm_r);
+                    } else {
+                        tile_points(t, nbt, r, rdr2, rdr);
+                        __syncthreads();
+                        tile_dl(nbt, dr, dr2);
+                        tile_h(nbt, 0, 0, M0, hm_r);
+                    }
                     __syncthreads();
                 }
 #pragma unroll
@@ -1542,7 +1611,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                                                -fmaf(hdf[3], tef[1], sg * hdf[1])};
                                 Q = pf_add(pf_fma<1>(Q, g0, g1, g2, gc, gr), PD);
                                 const Sp<float> E = delta_acc(Q, a, u);
-                                dsq = .25 * (double)fmaf(E.ar, E.ar, fmaf(E.ai, E.ai, fmaf(E.br, E.br, E.bi * E.bi)));
+                                dsq = (double)fmaf(E.ar, E.ar, fmaf(E.ai, E.ai, fmaf(E.br, E.br, E.bi * E.bi)));   // |E|^2
                             }
                             if constexpr (std::is_same<ST, float>::value) {
                                 const Sp<float> vf = candidate_t(a, dn, I.renorm);
