@@ -686,7 +686,7 @@ static int fill_info_from(const XfRunScalars& rs, xf_run_info* info, double ms) 
     if (rs.status == XF_EBARRIER && rs.diag_where)
         return xf_fail(rs.status, "stepper status %d (an exchange wait expired: workgroup %d, %s, waited for generation "
                                   "tag %u; %d rows missing, the first, row %d, holds tag bits %u)", rs.status, rs.diag_wg,
-                       rs.diag_where == 1 ? "poll" : "fetch", rs.diag_tag, rs.diag_missing, rs.diag_row, rs.diag_seen);
+                       rs.diag_where == 1 ? "first check" : "fetch", rs.diag_tag, rs.diag_missing, rs.diag_row, rs.diag_seen);
     if (rs.status)
         return xf_fail(rs.status, "stepper status %d (%s)", rs.status,
                        rs.status == XF_EBARRIER ? "an exchange wait expired"

@@ -786,14 +786,15 @@ __device__ __forceinline__ long long uni_ll(long long v) {
 //     reduction, and the wave publishes the total as row lb: value v ->
 //     its words (word v, or words 2v and 2v + 1), tagged with gen + 1
 //     (word_of), into every replica of rows[gen & 1][replica][word][nblk];
-//  2. wave 0 polls word 0 of all rows of replica lb % XF_ROW_REPLICAS (one
-//     contiguous array, each lane's loads in flight together) until every
-//     row carries the tag;
-//  3. each wave reads its values (two at a time) for all rows of that
-//     replica straight into registers (8 or 16 loads per lane in flight), checks
-//     every tag (re-reading on a stale one) and sums each value over the
-//     rows in a fixed order (lane l summing rows l, l + 64, ..., then the
-//     DPP reduction), which leaves the same bits in every workgroup.
+//  2. right after, each wave reads its values (two at a time) for all rows
+//     of replica lb % XF_ROW_REPLICAS straight into registers (8 or 16
+//     loads per lane in flight) and re-reads only the words whose tag is
+//     not yet current until every word is: the wait for the last row and
+//     the fetch are the same loads, so an exchange costs one round trip
+//     after the last row lands, and only the late rows are read again;
+//  3. each wave sums each value over the rows in a fixed order (lane l
+//     summing rows l, l + 64, ..., then the DPP reduction), which leaves
+//     the same bits in every workgroup.
 // Replicas spread the all-gather's reads, which every workgroup makes of the
 // same few kilobytes, over more memory lines.  The rows are the only data
 // that crosses workgroups, so no fence is needed: each word is one
@@ -802,8 +803,8 @@ __device__ __forceinline__ long long uni_ll(long long v) {
 // every row of gen + 1 arrived, and a workgroup publishes gen + 1 only after
 // its fetch of gen returned (the published values depend on the fetched
 // ones).  All waits are bounded.
-// Step 1 (xch_publish) and steps 2-4 (xch_gather), run back to back by
-// reduce_publish.
+// Step 1 (xch_publish) and steps 2-3 (xch_gather_spin, or xch_gather_poll
+// for the streaming modes), run back to back by reduce_publish.
 template <int NV, int NMAX, bool W1>
 __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int xcap, const XfInstDev& I, int lb,
                                             unsigned gen, long long* tr = nullptr) {
@@ -854,6 +855,104 @@ __device__ __forceinline__ void xch_publish(const double* acc, double* s_x, int 
     if (tr && tid == 0) tr[1] = (long long)__builtin_amdgcn_s_memrealtime();
 }
 
+// `idle`: work that needs none of the totals, run by each wave once its
+// first loads are in flight.
+// Every wave fetches its own values (two at a time) for all rows of its
+// replica, and a lane re-reads only the words whose tag is not yet current:
+// the wait for the last row and the fetch are one round trip.
+template <int NV, int NMAX, bool W1, class Idle>
+__device__ __forceinline__ bool xch_gather_spin(double* s_tot, const XfInstDev& I, int lb, unsigned& gen,
+                                           int* status, int* s_flag, long long* tr, Idle&& idle) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int nblk = I.nblk;
+    constexpr int NT = NV + NMAX;
+    constexpr int RP = XF_ROW_REPLICAS;
+    const unsigned tag = gen + 1u;
+    const unsigned long long tagw = word_of<W1>(0., tag, 0);             // a current word (for absent rows)
+    constexpr int WV = words_per_value<W1>();
+    const size_t rep_words = (size_t)XF_ROWW * nblk;
+    unsigned long long* rows = I.rows + (size_t)(gen & 1u) * RP * rep_words + (size_t)(lb % RP) * rep_words;
+    // failure flags alternate with the generation: the next exchange's is
+    // cleared here (it was last read before this exchange began)
+    if (tid == 0) s_flag[2 + ((gen + 1u) & 1u)] = 0;
+    const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
+    int wfail = 0;
+    bool idled = false;
+    for (int v0 = wave; v0 < NT; v0 += 2 * nwaves) {
+        const int v1 = v0 + nwaves;
+        const bool two = v1 < NT;                             // wave-uniform
+        double x[2] = {0., 0.};
+        for (int j0 = 0; j0 < nblk && !wfail; j0 += 256) {
+            unsigned long long w[2][WV][4];                   // [value][word][row]
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int q = 0; q < WV; ++q)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int j = j0 + lane + 64 * u;
+                        const int v = h ? v1 : v0;
+                        w[h][q][u] = ((h == 0 || two) && j < nblk) ? ld_word(rows + (size_t)(WV * v + q) * nblk + j) : tagw;
+                    }
+            if (!idled) {
+                idle();
+                idled = true;
+            }
+            for (int spins = 1;; ++spins) {
+                int bad = 0;
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int q = 0; q < WV; ++q)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) bad |= !tag_current<W1>(w[h][q][u], tag);
+                if (!__any(bad)) break;
+                if (const int e = wait_expired(t1, spins, status)) {
+                    if (e == 1) note_expiry<W1>(I.rs, 2, lb, tag, rows + (size_t)(WV * v0) * nblk, nblk);
+                    wfail = 1;
+                    break;
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int q = 0; q < WV; ++q)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (!tag_current<W1>(w[h][q][u], tag)) {
+                                const int j = j0 + lane + 64 * u;
+                                const int v = h ? v1 : v0;
+                                w[h][q][u] = ld_word(rows + (size_t)(WV * v + q) * nblk + j);
+                            }
+            }
+            if (wfail) break;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const bool is_sum = (h ? v1 : v0) < NV;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (j0 + lane + 64 * u >= nblk) continue;
+                    const double y = word_value<W1>(w[h][0][u], w[h][WV - 1][u]);
+                    x[h] = is_sum ? x[h] + y : fmax(x[h], y);
+                }
+            }
+        }
+        x[0] = (v0 < NV) ? wave_red_dpp<false>(x[0]) : wave_red_dpp<true>(x[0]);
+        if (lane == 63) s_tot[v0] = x[0];
+        if (two) {
+            x[1] = (v1 < NV) ? wave_red_dpp<false>(x[1]) : wave_red_dpp<true>(x[1]);
+            if (lane == 63) s_tot[v1] = x[1];
+        }
+    }
+    if (!idled) idle();
+    if (tr && tid == 0) tr[2] = (long long)__builtin_amdgcn_s_memrealtime();
+    if (wfail) s_flag[2 + (gen & 1u)] = 1;
+    __syncthreads();
+    const int fail = s_flag[2 + (gen & 1u)];
+    gen += 1u;
+    if (tr && tid == 0) tr[3] = (long long)__builtin_amdgcn_s_memrealtime();
+    return !fail;
+}
+
 // Whether word 0 of rows 0 .. n-1 (n <= 256) carries the tag: lane l reads
 // rows l + 64 u for u < 4, four loads in flight and none waiting for
 // another; indexes past the last row read the last row again.
@@ -873,10 +972,14 @@ __device__ __forceinline__ int rows_current(const unsigned long long* rows, int 
     return mine;
 }
 
-// `idle`: work that needs none of the totals, run by the other waves while
-// wave 0 polls (by wave 0 after its poll when the block is one wave).
+// The streaming modes' gather: a workgroup can finish its tiles well before
+// the others, and every wave re-reading its values for that long would load
+// the memory system, so wave 0 polls the tag words alone first (steps 2-3
+// above as a poll, then a fetch by every wave).  `idle`: work that needs none
+// of the totals, run by the other waves while wave 0 polls (by wave 0 after
+// its poll when the block is one wave).
 template <int NV, int NMAX, bool W1, class Idle>
-__device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, int lb, unsigned& gen,
+__device__ __forceinline__ bool xch_gather_poll(double* s_tot, const XfInstDev& I, int lb, unsigned& gen,
                                            int* status, int* s_flag, long long* tr, Idle&& idle) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int B = blockDim.x;
@@ -980,12 +1083,15 @@ __device__ __forceinline__ bool xch_gather(double* s_tot, const XfInstDev& I, in
 }
 
 struct NoIdle { __device__ void operator()() const {} };
-template <int NV, int NMAX, bool W1, class Idle = NoIdle>
+// SPIN: the resident modes, whose workgroups reach each exchange together
+// (xch_gather_spin); the streaming modes poll first (xch_gather_poll).
+template <int NV, int NMAX, bool W1, bool SPIN, class Idle = NoIdle>
 __device__ __forceinline__ bool reduce_publish(const double* acc, double* s_tot, double* s_x, int xcap, const XfInstDev& I,
                                                int lb, unsigned& gen, int* status, int* s_flag, long long* tr = nullptr,
                                                Idle&& idle = Idle{}) {
     xch_publish<NV, NMAX, W1>(acc, s_x, xcap, I, lb, gen, tr);
-    return xch_gather<NV, NMAX, W1>(s_tot, I, lb, gen, status, s_flag, tr, idle);
+    if constexpr (SPIN) return xch_gather_spin<NV, NMAX, W1>(s_tot, I, lb, gen, status, s_flag, tr, idle);
+    else return xch_gather_poll<NV, NMAX, W1>(s_tot, I, lb, gen, status, s_flag, tr, idle);
 }
 }  // namespace
 
@@ -1070,7 +1176,11 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         for (int q = 0; q < NS; ++q) put_src_weight<F64>(t + 6 + q, I.srcw[q * ne + e], I.srcwf[q * ne + e]);
     }
     __syncthreads();
-    __shared__ int s_flag[2];
+    __shared__ int s_flag[4];
+    if (tid == 0) {
+        s_flag[2] = 0;                                        // the exchanges' failure flags
+        s_flag[3] = 0;
+    }
     __shared__ double s_hm[2];                                // matter term at r + dr/2, r + dr
     // angular moments, uniform over the workgroup, gathered in place by the
     // exchanges: psi1 and psi3 (pass A), psi4 (pass B), and two buffers that
@@ -1238,7 +1348,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         // matter term: at r carried from the previous step (after an accepted
         // step r is the same double as its r + dr); at r + dr/2 and r + dr
         // (needed from pass B on) evaluated on lanes 0 and 1 of wave 1 while
-        // wave 0 polls in pass A's exchange
+        // its loads of pass A's exchange are in flight
         const double hm_r = hm_cur;
         double hm_rdr2 = 0., hm_rdr = 0.;
         auto hm_next = [&]() __attribute__((always_inline)) {
@@ -1346,7 +1456,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc + NMV, rec + R::W + 1 * NM, (double)Y0, (double)Y2, (double)Y3);  // h3 at r+dr/2
                 }
             }
-            ok = reduce_publish<2 * NMV, 0, !F64>(acc, s_MA, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr,
+            ok = reduce_publish<2 * NMV, 0, !F64, !STREAM>(acc, s_MA, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 1 : nullptr,
                                                   hm_next);
             if (!ok) break;
             hm_rdr2 = uni_d(s_hm[0]);
@@ -1438,7 +1548,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                     acc_moments<NM>(acc, rec + R::W + 2 * NM, (double)X0, (double)X2, (double)X3);   // h4 at r+dr
                 }
             }
-            ok = reduce_publish<NMV, 0, !F64>(acc, s_MB, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
+            ok = reduce_publish<NMV, 0, !F64, !STREAM>(acc, s_MB, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 5 : nullptr);
             if (!ok) break;
         }
 
@@ -1670,7 +1780,7 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
                 if (tr && tid == 0) tr[15] = (long long)__builtin_amdgcn_s_memrealtime();
             }
             acc[NMV] = blk_err;
-            ok = reduce_publish<NMV, 1, !F64>(acc, Mn, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
+            ok = reduce_publish<NMV, 1, !F64, !STREAM>(acc, Mn, s_errb, xcap, I, lb, gen, status, s_flag, tr ? tr + 9 : nullptr);
             errmax = uni_d(Mn[NMV]);
             if (!ok) break;
         }
