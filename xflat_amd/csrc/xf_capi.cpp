@@ -375,6 +375,14 @@ static int plan_shape(xf_ensemble* E, XfLaunch& L, bool commit) {
                 nblk[i] += 1;
                 if (nblk[i] < tiles[i]) busiest.push({(tiles[i] + nblk[i] - 1) / nblk[i], i});
             }
+            // as few workgroups as give the same busiest count, so that the
+            // instance's workgroups carry equal counts where the tiles divide
+            // (1000 tiles: 250 workgroups of 4 rather than 256 of 3 or 4),
+            // and none waits in an exchange for a whole tile's work
+            for (size_t i = 0; i < E->inst.size(); ++i) {
+                const int per = (tiles[i] + nblk[i] - 1) / nblk[i];
+                nblk[i] = (tiles[i] + per - 1) / per;
+            }
         }
         for (size_t i = 0; i < E->inst.size(); ++i) {
             grid += nblk[i];
