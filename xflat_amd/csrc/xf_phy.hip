@@ -303,8 +303,8 @@ int dalloc(XfPhy* p, double** q, size_t n) {
     void* v = nullptr;
     hipError_t e = hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(double));
     if (e != hipSuccess) return xf_fail(XF_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
-    hipMemset(v, 0, std::max<size_t>(n, 1) * sizeof(double));
     p->owned.push_back(v);
+    HIPCHK(hipMemset(v, 0, std::max<size_t>(n, 1) * sizeof(double)));
     *q = (double*)v;
     return XF_OK;
 }
