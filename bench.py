@@ -61,9 +61,9 @@ PROF = os.path.join(ROOT, "profiles")
 PROFILES = {
     ("ma1000x400", "fp32"): ((1, 4, 4, None), "k_step_fp32_ma_1000x400_hbm.md", "k_step_fp32_ma_1000x400_valu.md"),
     ("ma1000x400", "fp64"): ((4, 1, 1, None), "k_step_fp64_ma_1000x400.md", "k_step_fp64_ma_1000x400.md"),
-    ("ma1000x400", "mixed"): ((1, 4, 5, None), "k_step_mixed_ma_1000x400.md", "k_step_mixed_ma_1000x400.md"),
+    ("ma1000x400", "mixed"): ((1, 4, 4, None), "k_step_mixed_ma_1000x400.md", "k_step_mixed_ma_1000x400.md"),
     ("maa256x64x200", "fp32"): ((4, 2, 5, None), "k_step_fp32_maa_256x64x200.md", "k_step_fp32_maa_256x64x200.md"),
-    ("ensemble96", "fp32"): ((4, 1, None, 832), "k_step_fp32_ensemble96.md", "k_step_fp32_ensemble96.md"),
+    ("ensemble96", "fp32"): ((4, 4, None, 512), "k_step_fp32_ensemble96.md", "k_step_fp32_ensemble96.md"),
 }
 ACC_WINDOW = dict(R0=300.0, R1=340.0, points=20)
 
