@@ -1786,8 +1786,13 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         }
 
         // =================================================== decision (Nmr.cpp:195-249)
+        // the stop rule's parameters, read together (one batch of scalar
+        // loads instead of a chain of dependent ones)
+        const double eps0 = I.eps0, ksi = I.ksi, max_dr = I.max_dr, Rn = I.Rn;
+        const double dump_rstep = I.dump_rstep, dump_last_r = I.dump_last_r;
+        const int dump_itrstep = I.dump_itrstep;
         const double err = sqrt(errmax);
-        const bool acc_ok = err < I.eps0;
+        const bool acc_ok = err < eps0;
         if (lb == 0 && tid == 0 && I.log) {
             xf_step_rec rec;
             rec.r = r; rec.dr = dr; rec.err = err; rec.accepted = acc_ok ? 1 : 0; rec.pad = 0;
@@ -1823,13 +1828,13 @@ k_step(const XfInstDev* __restrict__ insts, const int* __restrict__ blk_inst, in
         ++steps_call;
         last_err = err;
         if (acc_ok) last_dr = dr;
-        dr *= I.ksi * sqrt(I.eps0 / err);
-        dr = (dr > I.max_dr) ? I.max_dr : dr;
-        dr = (r + dr > I.Rn) ? I.Rn - r : dr;
+        dr *= ksi * sqrt(eps0 / err);
+        dr = (dr > max_dr) ? max_dr : dr;
+        dr = (r + dr > Rn) ? Rn - r : dr;
         if (acc_ok) {
             bool due = false;
-            if (I.dump_rstep > 0. && (r == I.Rn || r - I.dump_last_r > I.dump_rstep)) due = true;
-            if (I.dump_itrstep > 0 && (it % I.dump_itrstep) == 0) due = true;
+            if (dump_rstep > 0. && (r == Rn || r - dump_last_r > dump_rstep)) due = true;
+            if (dump_itrstep > 0 && (it % dump_itrstep) == 0) due = true;
             if (due) { stop_reason = 2; break; }
         }
     }
