@@ -36,7 +36,7 @@ PEE_TOL = 1e-9
 # keeps dr within ~1e-10 of the reference's; the Phy mirror (gpuphy) takes
 # NBeam::U's sincos from the device library, within an ulp of glibc's, and its
 # dr on maa_pert.npz measured 1.0e-6 and 2.2e-6 from the reference's in two
-# runs on two GPU boxes.
+# runs on two different MI355X machines.
 DR_RTOL = {"gpu": 2e-6, "gpuphy": 1e-5}
 REF = os.path.join(ROOT, "oracle", "_ref")
 
