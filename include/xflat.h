@@ -258,7 +258,9 @@ int  xf_launch_info(const xf_ctx* ctx, int* vgprs, int* scratch, int* max_blocks
 /* Profiling: per-phase real-time stamps (100 MHz counter) of workgroup 0
  * for the first `steps` steps of the next xf_run_steps (16 slots per step:
  * [0] step start; [1..4] pass A: cells done, partial published, arrived,
- * totals known; [5..8] pass B; [9..12] pass C; [13] decision).  Arm with
+ * totals known; [5..8] pass B; [14] pass C's cells done, [15] its error
+ * sums done, [9..12] its exchange; [13] decision).  Tracing changes no
+ * result.  Arm with
  * xf_trace_next(ctx, buf, steps), run, then xf_trace_fetch(ctx, buf, steps).
  * No counterpart in the reference. */
 int  xf_trace_next(xf_ctx* ctx, long long* out, int steps);

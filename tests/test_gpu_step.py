@@ -382,6 +382,34 @@ def test_ensemble_instances_match_single_runs():
     ens.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_trace_stamps_are_ordered_and_change_nothing(prec):
+    """The profiling stamps (xf_trace_next / xf_trace_fetch, workgroup 0): a
+    traced run leaves the same state, r and dr, bit for bit, as the same run
+    untraced, and each step's stamps follow the step's order, the next step
+    starting after the decision."""
+    capi = _capi()
+    g, p = _golden("ma_r10_ih.npz")
+    pr = {"fp32": capi.XF_FP32, "fp64": capi.XF_FP64}[prec]
+    print(f"[trace] {prec}", flush=True)
+    out = []
+    for traced in (False, True):
+        with capi.Context(p, precision=pr) as c:
+            c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+            if traced:
+                tr, info = c.trace(30)
+            else:
+                info, _ = c.run(30)
+            out.append(c.download())
+    assert info.steps == 30
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1:] == out[1][1:]
+    order = [0, 1, 2, 3, 4, 5, 6, 7, 8, 14, 15, 9, 10, 11, 12, 13]
+    t = tr[:, order]
+    assert np.all(t > 0)
+    assert np.all(np.diff(t, axis=1) >= 0)
+    assert np.all(tr[1:, 0] >= tr[:-1, 13])
+
+
 def test_bad_launch_shapes_are_rejected():
     capi = _capi()
     from xflat_amd.config import bulb_params
