@@ -410,6 +410,32 @@ def test_trace_stamps_are_ordered_and_change_nothing(prec):
     assert np.all(tr[1:, 0] >= tr[:-1, 13])
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_autotune_restores_the_state(prec):
+    """xf_autotune times its candidate shapes on the caller's state and
+    restores it: the run after tuning equals, bit for bit, a run of the
+    chosen shape from the same state in a context that never tuned."""
+    capi = _capi()
+    g, p = _golden("ma_r50_ih.npz")
+    pr = {"fp32": capi.XF_FP32, "fp64": capi.XF_FP64}[prec]
+    print(f"[autotune] {prec}", flush=True)
+    with capi.Context(p, precision=pr) as c:
+        c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        L, ms = c.autotune(3)
+        assert ms > 0
+        assert c.download()[1] == float(g["r0"])
+        info_a, _ = c.run(25)
+        a = c.download()
+    with capi.Context(p, precision=pr) as c:
+        c.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        c.set_launch(L["mode"], L["cpt"], L["block"], L["beams_per_tile"])
+        assert c.launch() == L
+        info_b, _ = c.run(25)
+        b = c.download()
+    assert info_a.steps == info_b.steps == 25 and info_a.accepted == info_b.accepted
+    assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+
+
 def test_bad_launch_shapes_are_rejected():
     capi = _capi()
     from xflat_amd.config import bulb_params
