@@ -566,6 +566,39 @@ def test_full_size_fp64_matches_oracle_and_fp32_close(workload):
     assert d32 <= 1e-5
 
 
+def test_fp32_error_estimate_at_the_headline_size():
+    """MA 1000 x 400 from R = 10 km at eps0 = 1e-9: the FP32 stepper's error
+    estimate of one step from the same state, r and dr as the FP64 stepper's,
+    at 10 states along the FP64 run (every second attempt from dr = 1e-3).
+    It scatters around FP64's by the FP32 rounding of the cells' densities in
+    the sub-steps' moments: 3-5 % RMS, 15 % at worst (the seeded state
+    itself).  The bounds leave room for that scatter and not for a bias."""
+    capi = _capi()
+    from xflat_amd.config import bulb_params
+    from xflat_amd.seed import seeded_psi
+    p = bulb_params(Abins=1000, Ebins=400, R0=10, Rn=1000, eps0="1.E-9", dm2="-2.4E-3")
+    psi0 = seeded_psi(1000, 400, seed=2024, amp=1e-6)
+    print("[err estimate] MA 1000x400", flush=True)
+    states = []
+    with capi.Context(p, precision=capi.XF_FP64) as c:
+        c.upload(psi0, 10.0, 1e-3)
+        for _ in range(10):
+            states.append(c.download())
+            c.run(2)
+
+    def one(pr, st):
+        with capi.Context(p, precision=pr) as c:
+            c.upload(*st)
+            return c.run(1, log=True)[1]["err"][0]
+
+    e64 = np.array([one(capi.XF_FP64, st) for st in states])
+    e32 = np.array([one(capi.XF_FP32, st) for st in states])
+    rel = e32 / e64 - 1
+    print(f"[err estimate] err32/err64 - 1: mean {rel.mean():+.2e}, rms {np.sqrt((rel ** 2).mean()):.2e}, "
+          f"max {np.abs(rel).max():.2e}", flush=True)
+    assert abs(rel.mean()) < 0.04 and np.sqrt((rel ** 2).mean()) < 0.08 and np.abs(rel).max() < 0.25
+
+
 @pytest.mark.parametrize("module", ["ma", "sa"])
 def test_run_ends_exactly_at_rn(module):
     """The loop ends when r reaches Rn (Nmr.cpp:130); the last step is clamped
