@@ -156,7 +156,8 @@ def test_fused_modes_agree(prec):
     differently): FP64 takes the same accepted steps and keeps P_ee to the
     parity bar.  The FP32-increment steppers' error estimate scatters by a few
     per cent around FP64's on this grid at eps0 = 1e-8 (the FP32 rounding of
-    the remainder coefficients of grazing beams; pinned in
+    the remainder coefficients of grazing beams and of the cells' densities
+    in the sub-steps' moments; pinned in
     test_fp32_error_estimate_from_identical_states), and their states differ
     by FP32 rounding from the first step, so near-threshold decisions differ
     and then the dr sequences: the launches advance the same radius to within
@@ -200,7 +201,9 @@ def test_fp32_error_estimate_from_identical_states(name, rms, worst):
     FP64 trajectory, one attempted step each.  The FP32 estimate scatters
     around FP64's by the FP32 rounding of the second-order remainder
     coefficients, 6e-8 x^2 for x = lambda dl, which is largest for grazing
-    beams (long dl) near the neutrinosphere: a few per cent (absolute ~4e-10)
+    beams (long dl) near the neutrinosphere, and of the cells' densities in
+    the sub-steps' moments (6e-8 of each, which reaches the estimate through
+    the Hamiltonians' differences): a few per cent (absolute ~4e-10)
     at R = 10 km, eps0 = 1e-8 on the 32 x 16 grid, a few per mille at R = 50
     km, eps0 = 1e-9.  The mean deviation stays below 2 %, the RMS and the
     largest below the per-grid bounds; every launch shape of one stepper
