@@ -194,6 +194,34 @@ def test_fused_modes_agree(prec):
             assert dpee <= 1e-5, key
 
 
+@pytest.mark.parametrize("name,bpt", [("maa_r10.npz", 4), ("sa_r10_ih.npz", 1)])
+def test_fp64_launch_shapes_agree_on_maa_and_sa(name, bpt):
+    """FP64 on the MAA and SA geometries at every instantiated launch shape
+    (resident recompute, streaming, streaming with the records in LDS; 1, 2
+    and 4 cells per lane): each takes the reference's accepted-step sequence
+    and keeps P_ee / Pbar_ee within the parity bar of the reference's final
+    state, and the two streaming modes agree bit for bit."""
+    capi = _capi()
+    g, p = _golden(name)
+    total = int(g["total"])
+    print(f"[fp64 shapes] {name}", flush=True)
+    out = {}
+    for mode in (1, 2, 4):
+        for cpt in (1, 2, 4):
+            with capi.Context(p, precision=capi.XF_FP64) as ctx:
+                ctx.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+                ctx.set_launch(mode, cpt, capi.block_for(bpt, p.ebins, cpt), bpt)
+                rec = _run_all(ctx, total)
+                out[(mode, cpt)] = (rec, ctx.download()[0])
+    ge, gb = _pee(g["psi"][-1])
+    for key, (rec, psi) in out.items():
+        assert np.array_equal(np.nonzero(rec["accepted"])[0], g["it"]), key
+        pe, pb = _pee(psi)
+        assert max(np.abs(pe - ge).max(), np.abs(pb - gb).max()) <= PEE_TOL, key
+        if key[0] == 4:
+            assert np.array_equal(psi, out[(2, key[1])][1]), key
+
+
 @pytest.mark.parametrize("name,rms,worst", [("ma_r10_ih.npz", 0.08, 0.2), ("ma_r50_ih.npz", 0.01, 0.03)])
 def test_fp32_error_estimate_from_identical_states(name, rms, worst):
     """The compensated error estimate of the FP32-increment steppers against
