@@ -86,7 +86,8 @@ typedef struct xf_params {
  * renormalisation off, resp. forces the streaming modes). */
 typedef struct xf_opts {
     int precision;        /* XF_FP64 / XF_MIXED / XF_FP32                       */
-    int species;          /* 0 auto, 2 = SU(2)-partner reduced, 4 = explicit    */
+    int species;          /* 0 auto, 2 = SU(2)-partner reduced, 4 = explicit
+                             (FP64 and XF_MIXED; XF_EINVAL with XF_FP32)        */
     int renorm;           /* unitarity renormalisation of accepted states
                              (-1 auto: on for XF_MIXED/XF_FP32, off for FP64)   */
     int block;            /* threads per workgroup (64..1024), 0 auto          */
@@ -195,7 +196,9 @@ int  xf_calc_err_hvv(xf_ctx* ctx, int a, int b, double* max_err);
  * xf_state_upload takes a full 4-species psi (reference layout) plus the
  * radius and step length; with 2-species mode the nu_x / anti-nu_x columns
  * must be SU(2) partners (-conj(b), conj(a)) of nu_e / anti-nu_e, else
- * XF_ESTATE.  xf_state_download reconstructs all 4 species.               */
+ * XF_ESTATE (with species 0 in FP64 or XF_MIXED the context switches to 4
+ * species and takes the state instead).  xf_state_download reconstructs all 4
+ * species.                                                                */
 int  xf_state_upload(xf_ctx* ctx, const double* psi, double r, double dr);
 int  xf_state_init_pure(xf_ctx* ctx);            /* pure flavour, r = R0, dr = dr= */
 int  xf_state_download(xf_ctx* ctx, double* psi, double* r, double* dr);

@@ -441,6 +441,41 @@ def test_trace_stamps_are_ordered_and_change_nothing(prec):
     assert np.all(tr[1:, 0] >= tr[:-1, 13])
 
 
+def test_non_partner_states_by_precision():
+    """A state whose nu_x columns are not the SU(2) partners of nu_e's: the
+    FP64 and mixed steppers switch to 4 explicit species and step it (mixed
+    within 1e-5 of FP64 after 60 attempts); the FP32-state stepper, built for
+    partner states only, refuses it with XF_ESTATE, and an explicit
+    4-species FP32 context with XF_EINVAL."""
+    capi = _capi()
+    g, p = _golden("ma_r10_ih.npz")
+    psi = np.array(g["psi0"], dtype=np.float64)
+    psi[:, 2, 0] *= 0.999           # nu_x no longer the partner of nu_e
+    psi[:, 2, 2] = np.sqrt(np.maximum(0., 1. - psi[:, 2, 0] ** 2 - psi[:, 2, 1] ** 2 - psi[:, 2, 3] ** 2))
+    print("[non-partner]", flush=True)
+    with pytest.raises(capi.XfError) as e:
+        capi.Context(p, precision=capi.XF_FP32, species=4)
+    assert e.value.code == -1
+    with capi.Context(p, precision=capi.XF_FP32) as c:
+        with pytest.raises(capi.XfError) as e:
+            c.upload(psi, float(g["r0"]), float(g["dr0"]))
+        assert e.value.code == -4
+    out = {}
+    for name, pr in (("fp64", capi.XF_FP64), ("mixed", capi.XF_MIXED)):
+        with capi.Context(p, precision=pr) as c:
+            c.upload(psi, float(g["r0"]), float(g["dr0"]))
+            assert c.species() == 4
+            info, _ = c.run(60)
+            assert info.steps == 60
+            out[name] = c.download()
+    d = np.abs(out["mixed"][0] - out["fp64"][0]).max()
+    print(f"[non-partner] mixed vs fp64 after 60 attempts: max |dpsi| {d:.2e}", flush=True)
+    for k in range(4):
+        pm = out["mixed"][0][:, k, 0] ** 2 + out["mixed"][0][:, k, 1] ** 2
+        pf = out["fp64"][0][:, k, 0] ** 2 + out["fp64"][0][:, k, 1] ** 2
+        assert np.abs(pm - pf).max() <= 1e-5, k
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 def test_autotune_restores_the_state(prec):
     """xf_autotune times its candidate shapes on the caller's state and

@@ -659,9 +659,12 @@ static int ens_create(const xf_params* p, int n, const xf_opts* o, xf_ensemble**
     if (o) E->o = *o;
     if (E->o.precision < XF_FP64 || E->o.precision > XF_FP32) return xf_fail(XF_EINVAL, "precision %d", E->o.precision);
     int ns = E->o.species == 4 ? 4 : 2;
+    if (ns == 4 && E->o.precision == XF_FP32)
+        return xf_fail(XF_EINVAL, "explicit 4-species states are stepped in FP64 or mixed precision: the "
+                                  "FP32-state stepper takes SU(2)-partner states (species 2)");
     int ren = E->o.renorm;
-    if (ren < 0 || (ren == 0 && o && o->renorm == 0 && false)) ren = 0;
-    if (E->o.renorm == -1) ren = (E->o.precision != XF_FP64) ? 1 : 0;
+    if (ren == -1) ren = (E->o.precision != XF_FP64) ? 1 : 0;     // automatic
+    else if (ren < 0) ren = 0;
     E->geom = p[0].module;
     for (int i = 0; i < n; ++i) {
         if (p[i].module != E->geom) return xf_fail(XF_EINVAL, "all instances of an ensemble share the geometry module");
@@ -922,8 +925,8 @@ int xf_state_upload(xf_ctx* ctx, const double* psi, double r, double dr) {
     if (!ctx || !psi) return xf_fail(XF_EINVAL, "xf_state_upload: null argument");
     XfInst* in = ctx->ens->inst[0].get();
     int rc = in->upload(psi, r, dr);
-    if (rc == XF_ESTATE && ctx->o.species == 0) {
-        // not partners: fall back to the explicit 4-species state
+    if (rc == XF_ESTATE && ctx->o.species == 0 && ctx->o.precision != XF_FP32) {
+        // not partners: fall back to the explicit 4-species state (FP64, mixed)
         xf_opts o4 = ctx->o;
         o4.species = 4;
         xf_ensemble* E = nullptr;
