@@ -56,14 +56,16 @@ METRIC = "beam-steps/sec (angle x energy x dr) at stated P_ee error vs FP64; % H
 R0, DR0 = 10.0, 1e-3
 PROF = os.path.join(ROOT, "profiles")
 # committed rocprof summaries (HBM traffic file, SQ counter file) of the
-# launch shapes they were taken at: (workload, precision) -> (mode, cells per
-# lane, beams per tile or None, block or None)
+# launch shapes they were taken at: (workload, precision) -> list of ((mode,
+# cells per lane, beams per tile or None, block or None), HBM file, SQ file),
+# one per shape the autotuner picks on some boxes
 PROFILES = {
-    ("ma1000x400", "fp32"): ((1, 4, 4, None), "k_step_fp32_ma_1000x400_hbm.md", "k_step_fp32_ma_1000x400_valu.md"),
-    ("ma1000x400", "fp64"): ((4, 1, 1, None), "k_step_fp64_ma_1000x400.md", "k_step_fp64_ma_1000x400.md"),
-    ("ma1000x400", "mixed"): ((1, 4, 4, None), "k_step_mixed_ma_1000x400.md", "k_step_mixed_ma_1000x400.md"),
-    ("maa256x64x200", "fp32"): ((4, 2, 5, None), "k_step_fp32_maa_256x64x200.md", "k_step_fp32_maa_256x64x200.md"),
-    ("ensemble96", "fp32"): ((4, 4, None, 512), "k_step_fp32_ensemble96.md", "k_step_fp32_ensemble96.md"),
+    ("ma1000x400", "fp32"): [((1, 4, 4, None), "k_step_fp32_ma_1000x400_hbm.md", "k_step_fp32_ma_1000x400_valu.md"),
+                             ((0, 4, 4, None), "k_step_fp32_ma_1000x400_mode0.md", "k_step_fp32_ma_1000x400_mode0.md")],
+    ("ma1000x400", "fp64"): [((4, 1, 1, None), "k_step_fp64_ma_1000x400.md", "k_step_fp64_ma_1000x400.md")],
+    ("ma1000x400", "mixed"): [((1, 4, 4, None), "k_step_mixed_ma_1000x400.md", "k_step_mixed_ma_1000x400.md")],
+    ("maa256x64x200", "fp32"): [((4, 2, 5, None), "k_step_fp32_maa_256x64x200.md", "k_step_fp32_maa_256x64x200.md")],
+    ("ensemble96", "fp32"): [((4, 4, None, 512), "k_step_fp32_ensemble96.md", "k_step_fp32_ensemble96.md")],
 }
 ACC_WINDOW = dict(R0=300.0, R1=340.0, points=20)
 
@@ -400,15 +402,13 @@ def _profile_value(path, pattern):
 def _profile(workload, prec_name, launch):
     """(HBM file, SQ file) of the committed profiles when they were taken at
     this launch shape, else None."""
-    ent = PROFILES.get((workload, prec_name))
-    if not ent:
-        return None
-    (mode, cpt, bpt, block), hbm, valu = ent
-    if (launch["mode"], launch["cpt"]) != (mode, cpt):
-        return None
-    if (bpt is not None and launch.get("beams_per_tile") != bpt) or (block is not None and launch.get("block") != block):
-        return None
-    return os.path.join(PROF, hbm), os.path.join(PROF, valu)
+    for (mode, cpt, bpt, block), hbm, valu in PROFILES.get((workload, prec_name), []):
+        if (launch["mode"], launch["cpt"]) != (mode, cpt):
+            continue
+        if (bpt is not None and launch.get("beams_per_tile") != bpt) or (block is not None and launch.get("block") != block):
+            continue
+        return os.path.join(PROF, hbm), os.path.join(PROF, valu)
+    return None
 
 
 def _roofline(workload, prec_name, cells, steps, kernel_ms, launch):

@@ -26,10 +26,13 @@ def _dispatch_shape(text):
     return (int(m.group(1).replace(",", "")), int(m.group(2))) if m else (None, None)
 
 
-@pytest.mark.parametrize("key", sorted(bench.PROFILES))
-def test_profile_summaries_feed_the_bench_line(key):
+ENTRIES = [(key, i) for key in sorted(bench.PROFILES) for i in range(len(bench.PROFILES[key]))]
+
+
+@pytest.mark.parametrize("key,i", ENTRIES)
+def test_profile_summaries_feed_the_bench_line(key, i):
     workload, prec = key
-    shape, hbm, valu = bench.PROFILES[key]
+    shape, hbm, valu = bench.PROFILES[key][i]
     for f in (hbm, valu):
         assert os.path.exists(os.path.join(bench.PROF, f)), f
     text = open(os.path.join(bench.PROF, valu)).read() + open(os.path.join(bench.PROF, hbm)).read()
@@ -37,7 +40,7 @@ def test_profile_summaries_feed_the_bench_line(key):
     if grid is None:                     # the ensemble's summary gives the block only
         grid, block = 256, shape[3]
     launch = _launch(shape, grid, block)
-    assert bench._profile(workload, prec, launch) is not None
+    assert bench._profile(workload, prec, launch) == (os.path.join(bench.PROF, hbm), os.path.join(bench.PROF, valu))
 
     steps, kernel_ms = 100, 2.0
     cells = BEAMS[workload] or 14_200_000
@@ -58,7 +61,7 @@ def test_profile_summaries_feed_the_bench_line(key):
 def test_other_launch_shapes_get_no_profile_figures():
     """A launch shape the summaries were not taken at reports no traffic and
     no VALU roofline rather than another shape's numbers."""
-    shape, _, _ = bench.PROFILES[("ma1000x400", "fp32")]
+    shape, _, _ = bench.PROFILES[("ma1000x400", "fp32")][0]
     other = _launch((shape[0], 2 if shape[1] != 2 else 1, shape[2], shape[3]), 250, 448)
     assert bench._profile("ma1000x400", "fp32", other) is None
     roof = bench._roofline("ma1000x400", "fp32", 400000, 10, 1.0, other)
