@@ -357,6 +357,46 @@ def test_mixed_precision_close_to_fp64(prec, name):
     assert norm <= 1e-6
 
 
+@pytest.mark.parametrize("name", golden_files())
+@pytest.mark.parametrize("prec", ["mixed", "fp32"])
+def test_reduced_precision_at_equal_radius_on_every_golden(prec, name):
+    """Every reference fixture (MA, MAA, SA; both hierarchies; vacuum): the
+    mixed or FP32 stepper runs the fixture's attempts, and the FP64 stepper
+    is then stopped exactly on the radius it reached (Rn moved there, the
+    last step clamped as Nmr.cpp:249 clamps it), so both states are compared
+    at the same r: P_ee and Pbar_ee within 1e-5, unitarity within 1e-6."""
+    capi = _capi()
+    g, p = _golden(name)
+    pr = capi.XF_MIXED if prec == "mixed" else capi.XF_FP32
+    print(f"[equal r] {prec} {name}", flush=True)
+    psi0 = np.asarray(g["psi0"])
+    partners = max(np.abs(psi0[:, 2 + s, c] + sg * psi0[:, s, c2]).max()
+                   for s in (0, 1) for c, c2, sg in ((0, 2, 1), (1, 3, -1), (2, 0, -1), (3, 1, 1))) <= 1e-12
+    with capi.Context(p, precision=pr) as b:
+        if prec == "fp32" and not partners:
+            # the -DPERTERB fixture perturbs nu_x on its own: FP32 takes
+            # partner states only and says so
+            with pytest.raises(capi.XfError) as e:
+                b.upload(psi0, float(g["r0"]), float(g["dr0"]))
+            assert e.value.code == -4
+            return
+        b.upload(psi0, float(g["r0"]), float(g["dr0"]))
+        _run_all(b, int(g["total"]))
+        pb_, rb, _ = b.download()
+    with capi.Context(p, precision=capi.XF_FP64) as a:
+        a.upload(g["psi0"], float(g["r0"]), float(g["dr0"]))
+        n = a.run_to(rb)
+        pa, ra, _ = a.download()
+    assert ra == rb
+    (ea, ba), (eb, bb) = _pee(pa), _pee(pb_)
+    d = max(np.abs(ea - eb).max(), np.abs(ba - bb).max())
+    norm = max(np.abs(np.sum(pb_[:, k] ** 2, axis=1) - 1).max() for k in range(4))
+    print(f"[equal r] {prec} {name}: r = {rb:.9f} after {int(g['total'])} attempts (FP64 {n}), "
+          f"max |dP| {d:.2e}, unitarity {norm:.2e}", flush=True)
+    assert d <= 1e-5
+    assert norm <= 1e-6
+
+
 @pytest.mark.parametrize("prec", ["mixed", "fp32"])
 def test_mixed_step_sequence_follows_fp64(prec):
     """At eps0 = 1e-9 near the neutrinosphere the FP32 rounding of a first-order
